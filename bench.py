@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Throughput of the CGFD3D time-stepping hot path on one MI355X.
+
+Workload: the large single-GPU heterogeneous model -- 600x600x300 cells,
+dh = 50 m, random von Karman surface (a = 5 km, RMS 300 m), 1-D gradient
+Vp 3500 -> 7000 m/s with a 5 % von Karman perturbation, Gardner density,
+20-cell ADE CFS-PML on five faces, traction-image free surface, a finite
+fault of 200 double-couple subfaults, 400 surface receivers and PGV.  Inputs
+are synthetic (seeded) and resident in HBM before timing starts.
+
+A step is one full RK4 time step (4 stages) of the whole grid.  The metric
+is Gcell-RKstages/s = ni*nj*nk*4*steps / t.  The roofline entry divides the
+algorithmic bytes of one stage (192 B per cell + 144 B per PML cell-axis,
+FP32) by the mean duration of the fused stage kernel measured with HIP events
+on the engine's stream over the timed steps.  The CPU baseline is the oracle
+(a C/OpenMP restatement of the reference kernels) timed on this host on one
+step of the same problem.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "copara-cgfd3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(name, nt):
+    import scenario
+
+    if name == "vonkarman":
+        return scenario.config_vonkarman(nt=nt)
+    if name == "hill":
+        return scenario.config_hill(nt=nt)
+    if name == "flat":
+        return scenario.config_flat(nt=nt)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def describe(sc):
+    return {
+        "workload": sc["name"],
+        "cells": [sc["ni"], sc["nj"], sc["nk"]],
+        "dh_m": sc["dh"],
+        "dt_s": sc["dt"],
+        "pml": list(map(int, sc["nabs"])),
+        "sources": int(len(sc.get("moment_locs", [])) + len(sc.get("force_locs", []) if sc.get("force_locs") is not None else [])),
+        "receivers": int(len(sc["recv_locs"])),
+        "free_surface": "traction image",
+        "storage": "f32",
+        "stencil_accumulation": "f64",
+    }
+
+
+def cpu_baseline(sc, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    o.setup(sc)
+    t0 = time.perf_counter()
+    o.run(steps)
+    t = time.perf_counter() - t0
+    nv = sc["ni"] * sc["nj"] * sc["nk"]
+    val = nv * 4 * steps / t / 1e9
+    threads = oracle.num_threads()
+    o.close()
+    return {"value": round(val, 6), "unit": "Gcell-RKstages/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} step(s) = {4 * steps} RK stages of the full {sc['ni']}x{sc['nj']}x{sc['nk']} "
+                      f"problem on the C/OpenMP oracle, {threads} threads, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat"])
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--zchunks", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU decomposition is out of scope).", file=sys.stderr)
+        sys.exit(2)
+
+    import cgfd3d
+
+    nt = args.warmup + args.steps
+    t0 = time.perf_counter()
+    sc = build_workload(args.workload, nt)
+    log(f"workload {sc['name']} built in {time.perf_counter() - t0:.1f} s")
+    sim = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    if args.zchunks or args.rows:
+        sim.set_tuning(args.zchunks, args.rows)
+    t0 = time.perf_counter()
+    sim.setup(sc)
+    sim.sync()
+    log(f"inputs uploaded in {time.perf_counter() - t0:.1f} s")
+
+    sim.run(args.warmup)
+    sim.sync()
+    log(f"{args.warmup} warmup steps done")
+    sim.profile(True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    sim.sync()
+    t = time.perf_counter() - t0
+    prof = sim.profile_read()
+    sim.profile(False)
+    log(f"{args.steps} timed steps in {t:.3f} s")
+
+    nv = sc["ni"] * sc["nj"] * sc["nk"]
+    value = nv * 4 * args.steps / t / 1e9
+    stage_ms = prof["stage_ms"] / max(prof["stage_n"], 1)
+    bytes_stage = sim.stage_bytes()
+    achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
+    seis = sim.get_seismo()
+    finite = bool(abs(seis).max() < float("inf"))
+    sim.close()
+    del sim
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log("timing the CPU baseline")
+        cpu = cpu_baseline(sc, args.cpu_steps)
+        log(f"CPU baseline {cpu['value']:.4f} Gcell-RKstages/s on {cpu['cores']} threads")
+
+    cfg = describe(sc)
+    cfg.update({"stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
+                "bytes_per_stage_model": bytes_stage, "finite_output": finite})
+    line = {
+        "metric": "Gcell-RKstages/s",
+        "value": round(value, 4),
+        "unit": "Gcell-RKstages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,18 @@
+#!/bin/bash
+# Builds libcgfd3d.so (gfx950) in-tree.  Kernels are compiled without FP
+# contraction (bit-exact arithmetic contract), with IEEE denormals, and with
+# packed FP32 instructions disabled.
+set -euo pipefail
+here="$(cd "$(dirname "$0")" && pwd)"
+root="$(cd "$here/.." && pwd)"
+ARCH="${CGFD_ARCH:-gfx950}"
+FLAGS=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -ffp-contract=off -fno-gpu-flush-denormals-to-zero
+       -fhip-fp32-correctly-rounded-divide-sqrt
+       -Xclang -target-feature -Xclang -packed-fp32-ops
+       -I"$root/include" -I"$here/csrc")
+mkdir -p "$here/_build"
+hipcc "${FLAGS[@]}" ${CGFD_EXTRA_FLAGS:-} -c "$here/csrc/cgfd_kernels.hip" -o "$here/_build/cgfd_kernels.o"
+hipcc "${FLAGS[@]}" -c "$here/csrc/cgfd_capi.hip" -o "$here/_build/cgfd_capi.o"
+hipcc -shared -fPIC --offload-arch="$ARCH" -o "$here/libcgfd3d.so.tmp" "$here/_build/cgfd_kernels.o" "$here/_build/cgfd_capi.o"
+mv -f "$here/libcgfd3d.so.tmp" "$here/libcgfd3d.so"
+echo "built $here/libcgfd3d.so"

@@ -1,0 +1,265 @@
+"""ctypes binding of libcgfd3d.so, the gfx950 time-stepping engine.
+
+This is the host-side mirror of the reference's ChildProcs interface
+(src/typedata.h:114-217 of fes478/CoPara-CGFD3D) for Python callers: the same
+inputs (metrics, media, CFS-PML tables, point sources, receivers) in the
+reference's [x][y][z]+3-ghost layout, the same stepping entry points and the
+same outputs (receiver traces, peak ground velocity, wavefield).  All compute
+runs in the HIP library; loading fails loudly if it has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcgfd3d.so")
+
+FP32, FP64 = 0, 1
+COMPONENTS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
+METRICS = ("xix", "xiy", "xiz", "etax", "etay", "etaz", "zetax", "zetay", "zetaz", "jac")
+
+
+class CgfdError(RuntimeError):
+    pass
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int),
+                ("dh", ctypes.c_double), ("dt", ctypes.c_double), ("nt", ctypes.c_int)]
+
+
+_LIB = None
+
+# exported symbol -> (restype, argtypes); the list is the C-ABI of include/cgfd3d.h
+_VP = ctypes.c_void_p
+_IP = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "cgfd_create": (ctypes.c_int, [ctypes.POINTER(Dims), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "cgfd_destroy": (None, [_VP]),
+    "cgfd_last_error": (ctypes.c_char_p, [_VP]),
+    "cgfd_set_metrics": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+    "cgfd_set_media": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "cgfd_set_pml": (ctypes.c_int, [_VP, _IP, ctypes.POINTER(_VP)]),
+    "cgfd_set_forces": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _VP, _VP, ctypes.c_int]),
+    "cgfd_set_moments": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _VP, _VP, ctypes.c_int]),
+    "cgfd_set_receivers": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
+    "cgfd_set_pgv": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_init": (ctypes.c_int, [_VP]),
+    "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_sync": (ctypes.c_int, [_VP]),
+    "cgfd_current_step": (ctypes.c_int, [_VP]),
+    "cgfd_current_stage": (ctypes.c_int, [_VP]),
+    "cgfd_get_field": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
+    "cgfd_set_field": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
+    "cgfd_aux_size": (ctypes.c_long, [_VP, ctypes.c_int]),
+    "cgfd_get_aux": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
+    "cgfd_set_aux": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
+    "cgfd_get_seismo": (ctypes.c_int, [_VP, _VP]),
+    "cgfd_get_pgv": (ctypes.c_int, [_VP, _VP]),
+    "cgfd_get_cov": (ctypes.c_int, [_VP, _VP, _VP]),
+    "cgfd_profile": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_profile_read": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
+    "cgfd_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "cgfd_stage_bytes": (ctypes.c_double, [_VP]),
+    "cgfd_stream": (_VP, [_VP]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libcgfd3d.so; raises if it is missing (there is no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise CgfdError(f"{path} not built: run copara-cgfd3d_amd/build.sh (or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Simulation:
+    """One simulation context on one GPU (the reference's ChildProcs)."""
+
+    def __init__(self, ni, nj, nk, dh, dt, nt, precision=FP32, device=0):
+        self.lib = load_library()
+        self.ni, self.nj, self.nk, self.nt = ni, nj, nk, nt
+        self.dtype = np.float32 if precision == FP32 else np.float64
+        self.shape = (ni + 6, nj + 6, nk + 6)
+        dims = Dims(ni, nj, nk, float(dh), float(dt), nt)
+        h = ctypes.c_void_p()
+        rc = self.lib.cgfd_create(ctypes.byref(dims), precision, device, ctypes.byref(h))
+        if rc != 0:
+            raise CgfdError(f"cgfd_create failed with code {rc}")
+        self.h = h
+        self.nrecv = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cgfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.cgfd_last_error(self.h)
+            raise CgfdError(f"{what}: code {rc}: {msg.decode() if msg else ''}")
+
+    def _arr(self, a, shape=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if shape is not None and a.shape != shape:
+            raise CgfdError(f"expected shape {shape}, got {a.shape}")
+        return a
+
+    # ---- inputs -----------------------------------------------------------
+    def set_metrics(self, drv):
+        arrs = [self._arr(drv[k], self.shape) for k in METRICS]
+        ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrs])
+        self._check(self.lib.cgfd_set_metrics(self.h, ptrs), "set_metrics")
+
+    def set_media(self, alpha, beta, rho):
+        a, b, r = (self._arr(v, self.shape) for v in (alpha, beta, rho))
+        self._check(self.lib.cgfd_set_media(self.h, _ptr(a), _ptr(b), _ptr(r)), "set_media")
+
+    def set_pml(self, nabs, tabs):
+        """tabs: dict with APDx APDy APDz Bx By Bz DBx DBy DBz (lengths NX, NY, NZ)."""
+        order = ("APDx", "APDy", "APDz", "Bx", "By", "Bz", "DBx", "DBy", "DBz")
+        arrs = [self._arr(tabs[k]) for k in order]
+        nab = (ctypes.c_int * 6)(*[int(v) for v in nabs])
+        ptrs = (ctypes.c_void_p * 9)(*[a.ctypes.data for a in arrs])
+        self._check(self.lib.cgfd_set_pml(self.h, nab, ptrs), "set_pml")
+
+    def set_forces(self, locs, vec, stf):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        vec, stf = self._arr(np.reshape(vec, (n, 3))), self._arr(np.reshape(stf, (n, -1)))
+        self._check(self.lib.cgfd_set_forces(self.h, n, locs.ctypes.data_as(_IP), _ptr(vec), _ptr(stf),
+                                             stf.shape[1]), "set_forces")
+
+    def set_moments(self, locs, vec, stf):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        vec, stf = self._arr(np.reshape(vec, (n, 6))), self._arr(np.reshape(stf, (n, -1)))
+        self._check(self.lib.cgfd_set_moments(self.h, n, locs.ctypes.data_as(_IP), _ptr(vec), _ptr(stf),
+                                              stf.shape[1]), "set_moments")
+
+    def set_receivers(self, locs):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        self.nrecv = locs.shape[0]
+        self._check(self.lib.cgfd_set_receivers(self.h, self.nrecv, locs.ctypes.data_as(_IP)), "set_receivers")
+
+    def set_pgv(self, on=True):
+        self._check(self.lib.cgfd_set_pgv(self.h, 1 if on else 0), "set_pgv")
+
+    def init(self):
+        self._check(self.lib.cgfd_init(self.h), "init")
+
+    def setup(self, sc):
+        """Load every input of a scenario dict (see scenario.py) and init."""
+        self.set_metrics(sc["metrics"])
+        self.set_media(sc["alpha"], sc["beta"], sc["rho"])
+        self.set_pml(sc["nabs"], sc["pml"])
+        if sc.get("force_locs") is not None and len(sc["force_locs"]):
+            self.set_forces(sc["force_locs"], sc["force_vec"], sc["force_stf"])
+        if sc.get("moment_locs") is not None and len(sc["moment_locs"]):
+            self.set_moments(sc["moment_locs"], sc["moment_vec"], sc["moment_stf"])
+        if sc.get("recv_locs") is not None and len(sc["recv_locs"]):
+            self.set_receivers(sc["recv_locs"])
+        self.set_pgv(sc.get("pgv", True))
+        self.init()
+
+    # ---- stepping ---------------------------------------------------------
+    def run(self, nsteps):
+        self._check(self.lib.cgfd_run(self.h, int(nsteps)), "run")
+
+    def run_stages(self, nstages):
+        self._check(self.lib.cgfd_run_stages(self.h, int(nstages)), "run_stages")
+
+    def sync(self):
+        self._check(self.lib.cgfd_sync(self.h), "sync")
+
+    @property
+    def step(self):
+        return self.lib.cgfd_current_step(self.h)
+
+    @property
+    def stage(self):
+        return self.lib.cgfd_current_stage(self.h)
+
+    # ---- state and outputs -----------------------------------------------
+    def get_field(self, comp, which=0):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self._check(self.lib.cgfd_get_field(self.h, which, comp, _ptr(out)), "get_field")
+        return out
+
+    def set_field(self, comp, arr, which=0):
+        a = self._arr(arr, self.shape)
+        self._check(self.lib.cgfd_set_field(self.h, which, comp, _ptr(a)), "set_field")
+
+    def aux_size(self, axis):
+        return self.lib.cgfd_aux_size(self.h, axis)
+
+    def get_aux(self, axis, comp, which=0):
+        out = np.empty(max(self.aux_size(axis), 1), dtype=self.dtype)
+        self._check(self.lib.cgfd_get_aux(self.h, which, axis, comp, _ptr(out)), "get_aux")
+        return out[: self.aux_size(axis)]
+
+    def set_aux(self, axis, comp, arr, which=0):
+        a = self._arr(arr)
+        if a.size != self.aux_size(axis):
+            raise CgfdError("aux size mismatch")
+        self._check(self.lib.cgfd_set_aux(self.h, which, axis, comp, _ptr(a)), "set_aux")
+
+    def get_seismo(self):
+        out = np.zeros((9, self.nrecv, self.nt), dtype=self.dtype)
+        if self.nrecv:
+            self._check(self.lib.cgfd_get_seismo(self.h, _ptr(out)), "get_seismo")
+        return out
+
+    def get_pgv(self):
+        out = np.empty((3, self.shape[0], self.shape[1]), dtype=self.dtype)
+        self._check(self.lib.cgfd_get_pgv(self.h, _ptr(out)), "get_pgv")
+        return out
+
+    def get_cov(self):
+        cx = np.empty((self.shape[0], self.shape[1], 9), dtype=self.dtype)
+        cy = np.empty_like(cx)
+        self._check(self.lib.cgfd_get_cov(self.h, _ptr(cx), _ptr(cy)), "get_cov")
+        return cx, cy
+
+    # ---- measurement ------------------------------------------------------
+    def profile(self, on=True):
+        self._check(self.lib.cgfd_profile(self.h, 1 if on else 0), "profile")
+
+    def profile_read(self):
+        ms_s, n_s, ms_t, n_t = ctypes.c_double(), ctypes.c_long(), ctypes.c_double(), ctypes.c_long()
+        self._check(self.lib.cgfd_profile_read(self.h, ctypes.byref(ms_s), ctypes.byref(n_s), ctypes.byref(ms_t),
+                                               ctypes.byref(n_t)), "profile_read")
+        return {"stage_ms": ms_s.value, "stage_n": n_s.value, "timg_ms": ms_t.value, "timg_n": n_t.value}
+
+    def set_tuning(self, zchunks=0, rows=0):
+        self._check(self.lib.cgfd_set_tuning(self.h, int(zchunks), int(rows)), "set_tuning")
+
+    def stage_bytes(self):
+        return self.lib.cgfd_stage_bytes(self.h)

@@ -1,0 +1,857 @@
+// cgfd_capi.hip -- implementation of the C-ABI declared in include/cgfd3d.h.
+//
+// Owns every device buffer of one simulation (the role of ChildProcs,
+// src/typedata.h:114-217 of the reference), converts the reference's
+// [x][y][z]+ghost host arrays to the engine's padded x-fastest layout on the
+// GPU, and sequences the RK stages under the reference's forward/backward
+// schedule (src/simulation.cc:252-717).
+#include "cgfd3d.h"
+#include "cgfd_kernels.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace cgfd;
+
+namespace {
+
+constexpr int NSETS = 4; // FW, S1, S2, tW
+constexpr int SET_FW = 0, SET_S1 = 1, SET_S2 = 2, SET_TW = 3;
+
+long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+} // namespace
+
+struct cgfd_ctx {
+    int prec = CGFD_FP32;
+    size_t esz = 4;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    cgfd_dims d{};
+    Geom g{};
+    double h = 0, rh = 0;
+    long nref = 0; // NX*NY*NZ of the reference layout
+
+    char *sets = nullptr;
+    char *met = nullptr;
+    char *med = nullptr;
+    char *pml = nullptr;
+    char *staging = nullptr;
+    int nabs[6] = {0, 0, 0, 0, 0, 0};
+    int npml[3] = {0, 0, 0};
+    long astr[3] = {0, 0, 0};
+    char *aux[3] = {nullptr, nullptr, nullptr};
+    char *timg = nullptr;
+    long tstr = 0;
+    char *cov = nullptr;
+    long cstr = 0;
+    char *pgv = nullptr;
+    bool pgv_on = false;
+    bool have_metrics = false, have_media = false;
+
+    // point sources: host copies until init builds the device tables
+    int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
+    std::vector<int> floc, mloc;
+    std::vector<char> fvec, mvec, fstf, mstf;
+    int nent = 0;
+    int *col_first = nullptr;
+    int4 *sent = nullptr;
+    char *scoef = nullptr, *fstf_d = nullptr, *mstf_d = nullptr, *fvec_d = nullptr, *mvec_d = nullptr;
+
+    int nrecv = 0;
+    std::vector<int> rloc;
+    long *rlin = nullptr;
+    char *seis = nullptr;
+
+    int currT = 0, stage = 0;
+    bool inited = false;
+    int rows = 4, zchunks = 1;
+
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_stage, ev_timg;
+    double ms_stage = 0, ms_timg = 0;
+    long n_stage = 0, n_timg = 0;
+
+    std::string err;
+
+    char *set_ptr(int s) const { return sets + (size_t)s * NF * g.fs * esz; }
+    char *field_ptr(int s, int c) const { return set_ptr(s) + (size_t)c * g.fs * esz; }
+    char *aux_ptr(int axis, int s) const
+    {
+        return aux[axis] ? aux[axis] + (size_t)s * NF * astr[axis] * esz : nullptr;
+    }
+};
+
+static int fail(cgfd_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                      \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess)                                                                                  \
+            return fail(ctx, CGFD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+static int dalloc(cgfd_ctx *c, char **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, CGFD_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    // zero on the context's own stream (the guard cells of every field must
+    // read as zero) and wait, so later work on any stream sees it
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, CGFD_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    return 0;
+}
+
+static void dfree(char *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T> static void dfree_t(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// F/B signs of stage k of step t (simulation.cc:264-641): x and y share a
+// sign that alternates with the stage parity and flips for t%4 in {2,3}; z
+// alternates with the stage parity and flips for t%4 in {1,2}.
+static void fb_signs(int t, int k, int *sx, int *sz)
+{
+    const int p = t % 4;
+    *sx = (((p >= 2) ? 1 : 0) ^ (k & 1)) ? -1 : 1;
+    *sz = (((p == 1 || p == 2) ? 1 : 0) ^ (k & 1)) ? -1 : 1;
+}
+
+static void default_tuning(cgfd_ctx *c)
+{
+    c->rows = 4;
+    const long tiles = (long)((c->g.ni + 63) / 64) * ((c->g.nj + c->rows - 1) / c->rows);
+    long zc = (2048 + tiles - 1) / tiles;
+    zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 8)));
+    c->zchunks = (int)zc;
+}
+
+extern "C" {
+
+const char *cgfd_last_error(const cgfd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out)
+{
+    if (!out) return CGFD_EINVAL;
+    *out = nullptr;
+    if (!dims || dims->ni < 1 || dims->nj < 1 || dims->nk < 4 || dims->nt < 1 || !(dims->dh > 0) ||
+        !(dims->dt > 0) || (precision != CGFD_FP32 && precision != CGFD_FP64))
+        return CGFD_EINVAL;
+    cgfd_ctx *c = new cgfd_ctx();
+    c->prec = precision;
+    c->esz = precision == CGFD_FP64 ? 8 : 4;
+    c->dev = device;
+    c->d = *dims;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        fprintf(stderr, "cgfd_create: hipSetDevice(%d): %s\n", device, hipGetErrorString(e));
+        delete c;
+        return CGFD_EHIP;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CGFD_EHIP;
+    }
+    Geom &g = c->g;
+    g.ni = dims->ni;
+    g.nj = dims->nj;
+    g.nk = dims->nk;
+    g.nxp = (int)round_up(dims->ni + 3, 64);
+    g.nyp = dims->nj + 3;
+    g.plane = (long)g.nxp * g.nyp;
+    g.fs = round_up((long)(dims->nk + 8) * g.plane, 64);
+    g.base = 4 * g.plane;
+    c->nref = (long)(dims->ni + 6) * (dims->nj + 6) * (dims->nk + 6);
+    if (precision == CGFD_FP32) {
+        c->h = (double)(float)dims->dh;
+    } else {
+        c->h = dims->dh;
+    }
+    c->rh = 1.0 / c->h;
+    c->tstr = 3L * g.nj * g.nxp;
+    c->cstr = (long)g.nj * g.nxp;
+    int rc = 0;
+    if ((rc = dalloc(c, &c->sets, (size_t)NSETS * NF * g.fs * c->esz)) ||
+        (rc = dalloc(c, &c->met, (size_t)NDRV * g.fs * c->esz)) ||
+        (rc = dalloc(c, &c->med, (size_t)3 * g.fs * c->esz)) ||
+        (rc = dalloc(c, &c->pml, (size_t)3 * (g.ni + g.nj + g.nk) * c->esz)) ||
+        (rc = dalloc(c, &c->staging, (size_t)c->nref * c->esz)) ||
+        (rc = dalloc(c, &c->timg, (size_t)9 * c->tstr * c->esz)) ||
+        (rc = dalloc(c, &c->cov, (size_t)18 * c->cstr * c->esz)) ||
+        (rc = dalloc(c, &c->pgv, (size_t)3 * c->cstr * c->esz))) {
+        fprintf(stderr, "cgfd_create: %s\n", c->err.c_str());
+        cgfd_destroy(c);
+        return rc;
+    }
+    // unit scaling factors until cgfd_set_pml supplies tables
+    if (precision == CGFD_FP32) {
+        std::vector<float> t(3 * (g.ni + g.nj + g.nk), 0.0f);
+        for (int i = 0; i < g.ni; i++) t[g.ni + i] = 1;
+        for (int i = 0; i < g.nj; i++) t[3 * g.ni + g.nj + i] = 1;
+        for (int i = 0; i < g.nk; i++) t[3 * g.ni + 3 * g.nj + g.nk + i] = 1;
+        (void)hipMemcpy(c->pml, t.data(), t.size() * 4, hipMemcpyHostToDevice);
+    } else {
+        std::vector<double> t(3 * (g.ni + g.nj + g.nk), 0.0);
+        for (int i = 0; i < g.ni; i++) t[g.ni + i] = 1;
+        for (int i = 0; i < g.nj; i++) t[3 * g.ni + g.nj + i] = 1;
+        for (int i = 0; i < g.nk; i++) t[3 * g.ni + 3 * g.nj + g.nk + i] = 1;
+        (void)hipMemcpy(c->pml, t.data(), t.size() * 8, hipMemcpyHostToDevice);
+    }
+    default_tuning(c);
+    *out = c;
+    return 0;
+}
+
+static void free_sources(cgfd_ctx *c)
+{
+    dfree_t(c->col_first);
+    dfree_t(c->sent);
+    dfree(c->scoef);
+    dfree(c->fstf_d);
+    dfree(c->mstf_d);
+    dfree(c->fvec_d);
+    dfree(c->mvec_d);
+    c->nent = 0;
+}
+
+static void drop_events(cgfd_ctx *c)
+{
+    for (auto &p : c->ev_stage) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    for (auto &p : c->ev_timg) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    c->ev_stage.clear();
+    c->ev_timg.clear();
+}
+
+void cgfd_destroy(cgfd_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_events(c);
+    dfree(c->sets);
+    dfree(c->met);
+    dfree(c->med);
+    dfree(c->pml);
+    dfree(c->staging);
+    for (int a = 0; a < 3; a++) dfree(c->aux[a]);
+    dfree(c->timg);
+    dfree(c->cov);
+    dfree(c->pgv);
+    free_sources(c);
+    dfree_t(c->rlin);
+    dfree(c->seis);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// upload one reference-layout array into a padded field
+static int upload_field(cgfd_ctx *c, const void *src, char *dst)
+{
+    HIPCHK(c, hipMemcpyAsync(c->staging, src, (size_t)c->nref * c->esz, hipMemcpyHostToDevice, c->stream));
+    if (c->prec == CGFD_FP32)
+        launch_to_padded((const float *)c->staging, (float *)dst, c->g, c->stream);
+    else
+        launch_to_padded((const double *)c->staging, (double *)dst, c->g, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int download_field(cgfd_ctx *c, const char *src, void *dst)
+{
+    HIPCHK(c, hipMemsetAsync(c->staging, 0, (size_t)c->nref * c->esz, c->stream));
+    if (c->prec == CGFD_FP32)
+        launch_from_padded((const float *)src, (float *)c->staging, c->g, c->stream);
+    else
+        launch_from_padded((const double *)src, (double *)c->staging, c->g, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(dst, c->staging, (size_t)c->nref * c->esz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv)
+{
+    if (!c || !drv) return fail(c, CGFD_EINVAL, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    for (int k = 0; k < NDRV; k++) {
+        if (!drv[k]) return fail(c, CGFD_EINVAL, "null metric array");
+        int rc = upload_field(c, drv[k], c->met + (size_t)k * c->g.fs * c->esz);
+        if (rc) return rc;
+    }
+    c->have_metrics = true;
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_set_media(cgfd_ctx *c, const void *alpha, const void *beta, const void *rho)
+{
+    if (!c || !alpha || !beta || !rho) return fail(c, CGFD_EINVAL, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const void *src[3] = {alpha, beta, rho};
+    for (int k = 0; k < 3; k++) {
+        int rc = upload_field(c, src[k], c->med + (size_t)k * c->g.fs * c->esz);
+        if (rc) return rc;
+    }
+    c->have_media = true;
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_set_pml(cgfd_ctx *c, const int *nabs, const void *const *tabs)
+{
+    if (!c || !nabs || !tabs) return fail(c, CGFD_EINVAL, "null argument");
+    const int nv[3] = {c->g.ni, c->g.nj, c->g.nk};
+    for (int a = 0; a < 3; a++) {
+        if (nabs[2 * a] < 0 || nabs[2 * a + 1] < 0)
+            return fail(c, CGFD_EINVAL, "abs_number must be non-negative (absorb.cpp:151-157)");
+        if (nabs[2 * a] + nabs[2 * a + 1] >= nv[a])
+            return fail(c, CGFD_EINVAL, "total absorbing layers exceed the valid domain (absorb.cpp:158-163)");
+    }
+    for (int k = 0; k < 9; k++)
+        if (!tabs[k]) return fail(c, CGFD_EINVAL, "null PML table");
+    HIPCHK(c, hipSetDevice(c->dev));
+    // valid-index slices of the 9 tables, ordered APD B DB per axis
+    const int len[3] = {c->g.ni, c->g.nj, c->g.nk};
+    std::vector<char> host((size_t)3 * (c->g.ni + c->g.nj + c->g.nk) * c->esz);
+    size_t off = 0;
+    for (int a = 0; a < 3; a++)
+        for (int kind = 0; kind < 3; kind++) { // APD, B, DB
+            const char *src = (const char *)tabs[kind * 3 + a] + 3 * c->esz;
+            memcpy(host.data() + off, src, (size_t)len[a] * c->esz);
+            off += (size_t)len[a] * c->esz;
+        }
+    HIPCHK(c, hipMemcpy(c->pml, host.data(), host.size(), hipMemcpyHostToDevice));
+    memcpy(c->nabs, nabs, sizeof(c->nabs));
+    for (int a = 0; a < 3; a++) dfree(c->aux[a]);
+    c->npml[0] = nabs[0] + nabs[1];
+    c->npml[1] = nabs[2] + nabs[3];
+    c->npml[2] = nabs[4] + nabs[5];
+    c->astr[0] = round_up((long)c->g.nk * c->g.nj * c->npml[0], 64);
+    c->astr[1] = round_up((long)c->g.nk * c->npml[1] * c->g.nxp, 64);
+    c->astr[2] = round_up((long)c->npml[2] * c->g.nj * c->g.nxp, 64);
+    for (int a = 0; a < 3; a++) {
+        int rc = dalloc(c, &c->aux[a], (size_t)NSETS * NF * c->astr[a] * c->esz);
+        if (rc) return rc;
+    }
+    c->inited = false;
+    return 0;
+}
+
+static int check_points(cgfd_ctx *c, int n, const int *locs)
+{
+    for (int i = 0; i < n; i++) {
+        const int x = locs[3 * i], y = locs[3 * i + 1], z = locs[3 * i + 2];
+        if (x < 3 || x >= c->g.ni + 3 || y < 3 || y >= c->g.nj + 3 || z < 3 || z >= c->g.nk + 3)
+            return fail(c, CGFD_ERANGE,
+                        "point " + std::to_string(i) + " outside the valid domain (source.cpp:1077-1096)");
+    }
+    return 0;
+}
+
+int cgfd_set_forces(cgfd_ctx *c, int n, const int *locs, const void *vec, const void *stf, int nstf)
+{
+    if (!c || n < 0 || (n > 0 && (!locs || !vec || !stf))) return fail(c, CGFD_EINVAL, "bad force arguments");
+    if (n > 0 && nstf < 2 * c->d.nt + 1) return fail(c, CGFD_EINVAL, "force STF shorter than 2*nt+1 samples");
+    int rc = check_points(c, n, locs);
+    if (rc) return rc;
+    c->nfrc = n;
+    c->nstf_f = nstf;
+    c->floc.assign(locs, locs + 3 * n);
+    c->fvec.assign((const char *)vec, (const char *)vec + (size_t)3 * n * c->esz);
+    c->fstf.assign((const char *)stf, (const char *)stf + (size_t)n * nstf * c->esz);
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_set_moments(cgfd_ctx *c, int n, const int *locs, const void *vec, const void *stf, int nstf)
+{
+    if (!c || n < 0 || (n > 0 && (!locs || !vec || !stf))) return fail(c, CGFD_EINVAL, "bad moment arguments");
+    if (n > 0 && nstf < 2 * c->d.nt + 1) return fail(c, CGFD_EINVAL, "moment STF shorter than 2*nt+1 samples");
+    int rc = check_points(c, n, locs);
+    if (rc) return rc;
+    c->nmnt = n;
+    c->nstf_m = nstf;
+    c->mloc.assign(locs, locs + 3 * n);
+    c->mvec.assign((const char *)vec, (const char *)vec + (size_t)6 * n * c->esz);
+    c->mstf.assign((const char *)stf, (const char *)stf + (size_t)n * nstf * c->esz);
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_set_receivers(cgfd_ctx *c, int n, const int *locs)
+{
+    if (!c || n < 0 || (n > 0 && !locs)) return fail(c, CGFD_EINVAL, "bad receiver arguments");
+    int rc = check_points(c, n, locs);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->dev));
+    c->nrecv = n;
+    c->rloc.assign(locs, locs + 3 * n);
+    dfree_t(c->rlin);
+    dfree(c->seis);
+    if (n > 0) {
+        std::vector<long> lin(n);
+        for (int i = 0; i < n; i++) {
+            const int x = locs[3 * i] - 3, y = locs[3 * i + 1] - 3, z = locs[3 * i + 2] - 3;
+            lin[i] = c->g.base + ((long)z * c->g.nyp + y) * c->g.nxp + x;
+        }
+        HIPCHK(c, hipMalloc((void **)&c->rlin, n * sizeof(long)));
+        HIPCHK(c, hipMemcpy(c->rlin, lin.data(), n * sizeof(long), hipMemcpyHostToDevice));
+        rc = dalloc(c, &c->seis, (size_t)NF * n * c->d.nt * c->esz);
+        if (rc) return rc;
+    }
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_set_pgv(cgfd_ctx *c, int on)
+{
+    if (!c) return CGFD_EINVAL;
+    c->pgv_on = on != 0;
+    return 0;
+}
+
+// Source entries sorted by (column, z, force before moment, source index), so
+// that a lane meets the sources of its column in z order and applies
+// coincident ones in the reference's order (LoadForce before LoadMoment,
+// each in source-index order).
+static int build_sources(cgfd_ctx *c)
+{
+    free_sources(c);
+    struct Ent {
+        int col, z, kind, sid;
+    };
+    std::vector<Ent> ents;
+    for (int i = 0; i < c->nfrc; i++)
+        ents.push_back({(c->floc[3 * i + 1] - 3) * c->g.ni + (c->floc[3 * i] - 3), c->floc[3 * i + 2] - 3, 0, i});
+    for (int i = 0; i < c->nmnt; i++)
+        ents.push_back({(c->mloc[3 * i + 1] - 3) * c->g.ni + (c->mloc[3 * i] - 3), c->mloc[3 * i + 2] - 3, 1, i});
+    if (ents.empty()) return 0;
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        if (a.col != b.col) return a.col < b.col;
+        if (a.z != b.z) return a.z < b.z;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        return a.sid < b.sid;
+    });
+    const int n = (int)ents.size();
+    std::vector<int4> hs(n + 1);
+    std::vector<int> first((size_t)c->g.ni * c->g.nj, -1);
+    for (int e = 0; e < n; e++) {
+        hs[e] = make_int4(ents[e].z, ents[e].kind, ents[e].sid, ents[e].col);
+        if (first[ents[e].col] < 0) first[ents[e].col] = e;
+    }
+    hs[n] = make_int4(-1, -1, -1, -1);
+    HIPCHK(c, hipMalloc((void **)&c->sent, (n + 1) * sizeof(int4)));
+    HIPCHK(c, hipMemcpy(c->sent, hs.data(), (n + 1) * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc((void **)&c->col_first, first.size() * sizeof(int)));
+    HIPCHK(c, hipMemcpy(c->col_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = dalloc(c, &c->scoef, (size_t)n * 8 * c->esz))) return rc;
+    if (c->nfrc) {
+        if ((rc = dalloc(c, &c->fvec_d, c->fvec.size())) || (rc = dalloc(c, &c->fstf_d, c->fstf.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->fvec_d, c->fvec.data(), c->fvec.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->fstf_d, c->fstf.data(), c->fstf.size(), hipMemcpyHostToDevice));
+    }
+    if (c->nmnt) {
+        if ((rc = dalloc(c, &c->mvec_d, c->mvec.size())) || (rc = dalloc(c, &c->mstf_d, c->mstf.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->mvec_d, c->mvec.data(), c->mvec.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->mstf_d, c->mstf.data(), c->mstf.size(), hipMemcpyHostToDevice));
+    }
+    c->nent = n;
+    if (c->prec == CGFD_FP32)
+        launch_src_prep(c->g, (const float *)c->met, (const float *)c->med, c->sent, (const float *)c->fvec_d,
+                        (const float *)c->mvec_d, (float *)c->scoef, n, (float)c->h, c->stream);
+    else
+        launch_src_prep(c->g, (const double *)c->met, (const double *)c->med, c->sent, (const double *)c->fvec_d,
+                        (const double *)c->mvec_d, (double *)c->scoef, n, (double)c->h, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int cgfd_init(cgfd_ctx *c)
+{
+    if (!c) return CGFD_EINVAL;
+    if (!c->have_metrics || !c->have_media) return fail(c, CGFD_ESTATE, "metrics and media must be set before init");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemsetAsync(c->sets, 0, (size_t)NSETS * NF * c->g.fs * c->esz, c->stream));
+    for (int a = 0; a < 3; a++)
+        if (c->aux[a]) HIPCHK(c, hipMemsetAsync(c->aux[a], 0, (size_t)NSETS * NF * c->astr[a] * c->esz, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->pgv, 0, (size_t)3 * c->cstr * c->esz, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->timg, 0, (size_t)9 * c->tstr * c->esz, c->stream));
+    if (c->seis) HIPCHK(c, hipMemsetAsync(c->seis, 0, (size_t)NF * c->nrecv * c->d.nt * c->esz, c->stream));
+    if (c->prec == CGFD_FP32)
+        launch_velpd(c->g, (const float *)c->met, (const float *)c->med, (const float *)c->pml, (float *)c->cov,
+                     c->cstr, c->stream);
+    else
+        launch_velpd(c->g, (const double *)c->met, (const double *)c->med, (const double *)c->pml,
+                     (double *)c->cov, c->cstr, c->stream);
+    HIPCHK(c, hipGetLastError());
+    int rc = build_sources(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->currT = 0;
+    c->stage = 0;
+    c->inited = true;
+    return 0;
+}
+
+// input / output buffer sets of stage k: FW -> S1 -> S2 -> S1 -> FW
+static int stage_in(int k) { return k == 0 ? SET_FW : (k == 2 ? SET_S2 : SET_S1); }
+static int stage_out(int k) { return k == 0 ? SET_S1 : (k == 1 ? SET_S2 : (k == 2 ? SET_S1 : SET_FW)); }
+
+static void fold_events(cgfd_ctx *c)
+{
+    auto fold = [](std::vector<std::pair<hipEvent_t, hipEvent_t>> &v, double &ms, long &n) {
+        for (auto &p : v) {
+            float t = 0;
+            if (hipEventSynchronize(p.second) == hipSuccess && hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) {
+                ms += t;
+                n++;
+            }
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        v.clear();
+    };
+    fold(c->ev_stage, c->ms_stage, c->n_stage);
+    fold(c->ev_timg, c->ms_timg, c->n_timg);
+}
+
+} // extern "C"
+
+template <typename R> static int run_stage(cgfd_ctx *c)
+{
+    static const R RK4A[3] = {(R)0.5, (R)0.5, (R)1.0};                          // calculate.cu:12
+    static const R RK4B[4] = {(R)(1.0 / 6.0), (R)(1.0 / 3.0), (R)(1.0 / 3.0), (R)(1.0 / 6.0)}; // calculate.cu:13
+    const int k = c->stage;
+    int sx, sz;
+    fb_signs(c->currT, k, &sx, &sz);
+    const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
+    const R dt = (R)c->d.dt;
+    const int sin_ = stage_in(k), sout = stage_out(k);
+
+    TimgArgs<R> ta{};
+    ta.g = c->g;
+    ta.win = (const R *)c->set_ptr(sin_);
+    ta.met = (const R *)c->met;
+    ta.med = (const R *)c->med;
+    ta.pml = (const R *)c->pml;
+    memcpy(ta.nabs, c->nabs, sizeof(ta.nabs));
+    for (int a = 0; a < 2; a++) {
+        ta.npml[a] = c->npml[a];
+        ta.astr[a] = c->astr[a];
+        ta.ain[a] = (const R *)c->aux_ptr(a, sin_);
+    }
+    ta.timg = (R *)c->timg;
+    ta.tstr = c->tstr;
+    ta.h = c->h;
+
+    StageArgs<R> sa{};
+    sa.g = c->g;
+    sa.win = (const R *)c->set_ptr(sin_);
+    sa.fw = (const R *)c->set_ptr(SET_FW);
+    sa.wout = (R *)c->set_ptr(sout);
+    sa.tw = (R *)c->set_ptr(SET_TW);
+    sa.met = (const R *)c->met;
+    sa.med = (const R *)c->med;
+    sa.pml = (const R *)c->pml;
+    memcpy(sa.nabs, c->nabs, sizeof(sa.nabs));
+    for (int a = 0; a < 3; a++) {
+        sa.npml[a] = c->npml[a];
+        sa.astr[a] = c->astr[a];
+        sa.ain[a] = (const R *)c->aux_ptr(a, sin_);
+        sa.af[a] = (const R *)c->aux_ptr(a, SET_FW);
+        sa.aout[a] = (R *)c->aux_ptr(a, sout);
+        sa.at[a] = (R *)c->aux_ptr(a, SET_TW);
+    }
+    sa.timg = (const R *)c->timg;
+    sa.tstr = c->tstr;
+    sa.cov = (const R *)c->cov;
+    sa.cstr = c->cstr;
+    sa.col_first = c->nent ? c->col_first : nullptr;
+    sa.sent = c->sent;
+    sa.scoef = (const R *)c->scoef;
+    sa.fstf = (const R *)c->fstf_d;
+    sa.mstf = (const R *)c->mstf_d;
+    sa.nstf_f = c->nstf_f;
+    sa.nstf_m = c->nstf_m;
+    sa.tindex = 2 * c->currT + tinc;
+    sa.pgv = (k == 3 && c->pgv_on) ? (R *)c->pgv : nullptr;
+    sa.h = c->h;
+    sa.rh = c->rh;
+    sa.hr = (R)c->h;
+    sa.alpha = k < 3 ? RK4A[k] * dt : (R)0;
+    sa.beta = RK4B[k] * dt;
+    sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
+
+    // profiling: events bracket each launch on the context's stream
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (c->prof) {
+        for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+        HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    }
+    launch_timg<R>(sx, sz, ta, c->rows, c->stream);
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(ev[1], c->stream));
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    }
+    launch_stage<R>(sx, sz, k, sa, c->rows, c->zchunks, c->stream);
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+        c->ev_timg.push_back({ev[0], ev[1]});
+        c->ev_stage.push_back({ev[2], ev[3]});
+    }
+    HIPCHK(c, hipGetLastError());
+    if (k == 3) {
+        if (c->currT < c->d.nt && c->nrecv)
+            launch_record<R>((const R *)c->set_ptr(SET_FW), c->g.fs, c->rlin, c->nrecv, (R *)c->seis, c->d.nt,
+                             c->currT, c->stream);
+        HIPCHK(c, hipGetLastError());
+        c->currT++;
+        c->stage = 0;
+    } else {
+        c->stage = k + 1;
+    }
+    if (c->prof && c->ev_stage.size() >= 1024) fold_events(c);
+    return 0;
+}
+
+extern "C" {
+
+int cgfd_run_stages(cgfd_ctx *c, int nstages)
+{
+    if (!c || nstages < 0) return CGFD_EINVAL;
+    if (!c->inited) return fail(c, CGFD_ESTATE, "cgfd_init must run after the last set_* call");
+    HIPCHK(c, hipSetDevice(c->dev));
+    if ((long)c->currT * 4 + c->stage + nstages > 4L * c->d.nt)
+        return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
+    for (int s = 0; s < nstages; s++) {
+        int rc = c->prec == CGFD_FP32 ? run_stage<float>(c) : run_stage<double>(c);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int cgfd_run(cgfd_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return CGFD_EINVAL;
+    if (c->stage != 0) return fail(c, CGFD_ESTATE, "cgfd_run called in the middle of a step");
+    return cgfd_run_stages(c, 4 * nsteps);
+}
+
+int cgfd_sync(cgfd_ctx *c)
+{
+    if (!c) return CGFD_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cgfd_current_step(const cgfd_ctx *c) { return c ? c->currT : -1; }
+int cgfd_current_stage(const cgfd_ctx *c) { return c ? c->stage : -1; }
+
+// which: 0 = FW, 1 = W (output of the last stage), 2 = tW
+static int which_set(const cgfd_ctx *c, int which)
+{
+    if (which == 0) return SET_FW;
+    if (which == 2) return SET_TW;
+    if (which == 1) return c->stage == 0 ? SET_FW : stage_out(c->stage - 1);
+    return -1;
+}
+
+int cgfd_get_field(cgfd_ctx *c, int which, int comp, void *out)
+{
+    if (!c || !out || comp < 0 || comp >= NF) return fail(c, CGFD_EINVAL, "bad field request");
+    const int s = which_set(c, which);
+    if (s < 0) return fail(c, CGFD_EINVAL, "which must be 0, 1 or 2");
+    HIPCHK(c, hipSetDevice(c->dev));
+    return download_field(c, c->field_ptr(s, comp), out);
+}
+
+int cgfd_set_field(cgfd_ctx *c, int which, int comp, const void *in)
+{
+    if (!c || !in || comp < 0 || comp >= NF) return fail(c, CGFD_EINVAL, "bad field request");
+    if (which != 0 && which != 2) return fail(c, CGFD_EINVAL, "only FW (0) and tW (2) can be set");
+    if (!c->inited) return fail(c, CGFD_ESTATE, "set_field needs an initialised context");
+    HIPCHK(c, hipSetDevice(c->dev));
+    return upload_field(c, in, c->field_ptr(which_set(c, which), comp));
+}
+
+long cgfd_aux_size(const cgfd_ctx *c, int axis)
+{
+    if (!c || axis < 0 || axis > 2) return -1;
+    const long NX = c->g.ni + 6, NY = c->g.nj + 6, NZ = c->g.nk + 6;
+    if (axis == 0) return (long)c->npml[0] * NY * NZ;
+    if (axis == 1) return (long)c->npml[1] * NX * NZ;
+    return (long)c->npml[2] * NX * NY;
+}
+
+// index of the engine's auxiliary entry for reference packed entry (p, u, v)
+static long aux_engine_index(const cgfd_ctx *c, int axis, int pidx, int a, int b)
+{
+    // axis 0: (px, y, z); axis 1: (py, x, z); axis 2: (pz, x, y), all 0-based valid
+    if (axis == 0) return ((long)b * c->g.nj + a) * c->npml[0] + pidx;
+    if (axis == 1) return ((long)b * c->npml[1] + pidx) * c->g.nxp + a;
+    return ((long)pidx * c->g.nj + b) * c->g.nxp + a;
+}
+
+static int aux_xfer(cgfd_ctx *c, int which, int axis, int comp, void *host, bool get)
+{
+    if (!c || !host || axis < 0 || axis > 2 || comp < 0 || comp >= NF) return fail(c, CGFD_EINVAL, "bad aux request");
+    const int s = which_set(c, which);
+    if (s < 0) return fail(c, CGFD_EINVAL, "which must be 0, 1 or 2");
+    if (!c->aux[axis]) return 0;
+    HIPCHK(c, hipSetDevice(c->dev));
+    const long n = c->astr[axis];
+    std::vector<char> buf((size_t)n * c->esz);
+    char *dev = c->aux_ptr(axis, s) + (size_t)comp * n * c->esz;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (get) HIPCHK(c, hipMemcpy(buf.data(), dev, buf.size(), hipMemcpyDeviceToHost));
+    const int NX = c->g.ni + 6, NY = c->g.nj + 6, NZ = c->g.nk + 6;
+    const int na = axis == 0 ? c->g.nj : c->g.ni;          // second packed index (valid)
+    const int nb = axis == 2 ? c->g.nj : c->g.nk;          // third packed index (valid)
+    const int Ma = axis == 0 ? NY : NX, Mb = axis == 2 ? NY : NZ;
+    if (get) memset(host, 0, (size_t)cgfd_aux_size(c, axis) * c->esz);
+    for (int p = 0; p < c->npml[axis]; p++)
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                const long ref = ((long)p * Ma + a + 3) * Mb + b + 3;
+                const long eng = aux_engine_index(c, axis, p, a, b);
+                if (get)
+                    memcpy((char *)host + ref * c->esz, buf.data() + eng * c->esz, c->esz);
+                else
+                    memcpy(buf.data() + eng * c->esz, (const char *)host + ref * c->esz, c->esz);
+            }
+    if (!get) HIPCHK(c, hipMemcpy(dev, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cgfd_get_aux(cgfd_ctx *c, int which, int axis, int comp, void *out)
+{
+    return aux_xfer(c, which, axis, comp, out, true);
+}
+
+int cgfd_set_aux(cgfd_ctx *c, int which, int axis, int comp, const void *in)
+{
+    if (c && which != 0 && which != 2) return fail(c, CGFD_EINVAL, "only FA (0) and tA (2) can be set");
+    if (c && !c->inited) return fail(c, CGFD_ESTATE, "set_aux needs an initialised context");
+    return aux_xfer(c, which, axis, comp, const_cast<void *>(in), false);
+}
+
+int cgfd_get_seismo(cgfd_ctx *c, void *out)
+{
+    if (!c || !out) return CGFD_EINVAL;
+    if (!c->nrecv) return 0;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->seis, (size_t)NF * c->nrecv * c->d.nt * c->esz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// maps stored per valid (x,y) as [slot][nj][nxp] -> reference [slot][NX][NY]
+static int get_xy_maps(cgfd_ctx *c, const char *dev, int nslots, int slot_minor, void *out)
+{
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<char> buf((size_t)nslots * c->cstr * c->esz);
+    HIPCHK(c, hipMemcpy(buf.data(), dev, buf.size(), hipMemcpyDeviceToHost));
+    const long NX = c->g.ni + 6, NY = c->g.nj + 6;
+    memset(out, 0, (size_t)nslots * NX * NY * c->esz);
+    for (int s = 0; s < nslots; s++)
+        for (int x = 0; x < c->g.ni; x++)
+            for (int y = 0; y < c->g.nj; y++) {
+                const long src = (long)s * c->cstr + (long)y * c->g.nxp + x;
+                const long xy = (long)(x + 3) * NY + (y + 3);
+                const long dst = slot_minor ? xy * nslots + s : (long)s * NX * NY + xy;
+                memcpy((char *)out + dst * c->esz, buf.data() + src * c->esz, c->esz);
+            }
+    return 0;
+}
+
+int cgfd_get_pgv(cgfd_ctx *c, void *out)
+{
+    if (!c || !out) return CGFD_EINVAL;
+    return get_xy_maps(c, c->pgv, 3, 0, out);
+}
+
+int cgfd_get_cov(cgfd_ctx *c, void *covx, void *covy)
+{
+    if (!c || !covx || !covy) return CGFD_EINVAL;
+    int rc = get_xy_maps(c, c->cov, 9, 1, covx);
+    if (rc) return rc;
+    return get_xy_maps(c, c->cov + (size_t)9 * c->cstr * c->esz, 9, 1, covy);
+}
+
+int cgfd_profile(cgfd_ctx *c, int on)
+{
+    if (!c) return CGFD_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    fold_events(c);
+    c->prof = on != 0;
+    c->ms_stage = c->ms_timg = 0;
+    c->n_stage = c->n_timg = 0;
+    return 0;
+}
+
+int cgfd_profile_read(cgfd_ctx *c, double *ms_stage, long *n_stage, double *ms_timg, long *n_timg)
+{
+    if (!c) return CGFD_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    fold_events(c);
+    if (ms_stage) *ms_stage = c->ms_stage;
+    if (n_stage) *n_stage = c->n_stage;
+    if (ms_timg) *ms_timg = c->ms_timg;
+    if (n_timg) *n_timg = c->n_timg;
+    return 0;
+}
+
+int cgfd_set_tuning(cgfd_ctx *c, int zchunks, int rows)
+{
+    if (!c) return CGFD_EINVAL;
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(c, CGFD_EINVAL, "rows must be 1, 2 or 4");
+    if (zchunks < 0 || zchunks > c->g.nk) return fail(c, CGFD_EINVAL, "zchunks out of range");
+    if (rows) c->rows = rows;
+    if (zchunks) c->zchunks = zchunks;
+    return 0;
+}
+
+double cgfd_stage_bytes(const cgfd_ctx *c)
+{
+    if (!c) return -1;
+    const double nv = (double)c->g.ni * c->g.nj * c->g.nk;
+    const double npml = (double)c->npml[0] * c->g.nj * c->g.nk + (double)c->npml[1] * c->g.ni * c->g.nk +
+                        (double)c->npml[2] * c->g.ni * c->g.nj;
+    const double scale = c->esz / 4.0;
+    return scale * (192.0 * nv + 144.0 * npml);
+}
+
+void *cgfd_stream(const cgfd_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+} // extern "C"

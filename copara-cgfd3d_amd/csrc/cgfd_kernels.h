@@ -1,0 +1,74 @@
+// cgfd_kernels.h -- argument blocks and launchers of the gfx950 kernels.
+// Included by the kernel translation unit and by the C-ABI implementation.
+#pragma once
+#include "cgfd_device.h"
+
+namespace cgfd {
+
+// Arguments of one fused RK stage (RKite, calculate.cu:1355-1599).
+// Buffer sets hold the 9 wavefield components at stride g.fs.
+template <typename R> struct StageArgs {
+    Geom g;
+    const R *win; // stage input (FW for stage 0, else the previous stage output)
+    const R *fw;  // step-start field FW (read by stages 1 and 2)
+    R *wout;      // stage output (FW itself in stage 3)
+    R *tw;        // RK accumulator
+    const R *met; // 10 metric fields at stride g.fs
+    const R *med; // alpha, beta, rho at stride g.fs
+    const R *pml; // APD B DB for x (ni each), then y (nj each), then z (nk each)
+    int nabs[6];
+    int npml[3];   // packed PML widths per axis
+    long astr[3];  // stride between the 9 components of one auxiliary set
+    const R *ain[3];
+    const R *af[3];
+    R *aout[3];
+    R *at[3];
+    const R *timg; // traction-image results for the top 3 rows, 9 slots
+    long tstr;
+    const R *cov;  // free-surface closure matrices, 18 slots of [nj][nxp]
+    long cstr;
+    const int *col_first; // per (y*ni+x) column: first source entry or -1
+    const int4 *sent;     // source entries (z, kind, source id, column), sentinel column -1
+    const R *scoef;       // 8 coefficients per entry
+    const R *fstf;
+    const R *mstf;
+    int nstf_f, nstf_m;
+    int tindex;
+    R *pgv; // 3 slots of [nj][nxp] or null
+    double h, rh;
+    R hr;
+    R alpha, beta;
+    int zlen;
+};
+
+template <typename R> struct TimgArgs {
+    Geom g;
+    const R *win;
+    const R *met;
+    const R *med;
+    const R *pml;
+    int nabs[6];
+    int npml[2];
+    long astr[2];
+    const R *ain[2];
+    R *timg;
+    long tstr;
+    double h;
+};
+
+template <typename R>
+void launch_stage(int sx, int sz, int st, const StageArgs<R> &a, int rows, int zchunks, hipStream_t s);
+template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s);
+template <typename R>
+void launch_velpd(const Geom &g, const R *met, const R *med, const R *pml, R *cov, long cstr, hipStream_t s);
+template <typename R>
+void launch_to_padded(const R *src, R *dst, const Geom &g, hipStream_t s);
+template <typename R>
+void launch_from_padded(const R *src, R *dst, const Geom &g, hipStream_t s);
+template <typename R>
+void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent, const R *fvec, const R *mvec,
+                     R *scoef, int nent, R hr, hipStream_t s);
+template <typename R>
+void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s);
+
+} // namespace cgfd

@@ -1,0 +1,142 @@
+/*
+ * cgfd3d.h -- C-ABI of the MI355X (gfx950) time-stepping engine for the
+ * CGFD3D curvilinear-grid elastic-wave solver.
+ *
+ * The reference (fes478/CoPara-CGFD3D) has no plugin or FFI layer: its hot
+ * path sits behind the C++ class ChildProcs (src/typedata.h:114-217), which
+ * owns the device buffers, receives the host-prepared parameter arrays
+ * (ParH2D, src/calculate.cu:2050-2228), computes the free-surface closure
+ * (VelCoeff, calculate.cu:1336-1353), advances one RK stage per call (RKite,
+ * calculate.cu:1355-1599) under the forward/backward schedule of
+ * src/simulation.cc:252-717, and hands receiver traces and peak ground
+ * velocity back to the host (PWpick calculate.cu:2780-2828, SynPV 1709-1743).
+ * Each entry point below names the member it replaces.
+ *
+ * Conventions
+ *  - All functions return 0 on success and a negative CGFD_E* code on
+ *    failure; cgfd_last_error() gives the message.  Nothing aborts.
+ *  - Host arrays are borrowed for the duration of the call only.
+ *  - Array element type is `float` for a CGFD_FP32 context and `double` for a
+ *    CGFD_FP64 context (the reference's Real, typenew.h:13-30); `void*`
+ *    arguments below point to that type.
+ *  - 3-D arrays use the reference layout: flat [x][y][z], z fastest, with 3
+ *    ghost cells on each side: NX=ni+6, NY=nj+6, NZ=nk+6, valid indices
+ *    [3, n+3); the free surface is the last valid z row (z = nk+2).
+ *  - Point locations (sources, receivers) are ghost-inclusive (x,y,z)
+ *    triples, as the reference's locx/locy/locz (calculate.cu:6146-6148).
+ *  - Wavefield component order: Vx Vy Vz Txx Tyy Tzz Txy Txz Tyz
+ *    (typenew.h:265-270).
+ *  - One context drives one GPU from one host thread on one HIP stream.
+ */
+#ifndef CGFD3D_H
+#define CGFD3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGFD_FP32 0
+#define CGFD_FP64 1
+
+#define CGFD_OK 0
+#define CGFD_EINVAL (-1)    /* bad argument or inconsistent configuration */
+#define CGFD_ESTATE (-2)    /* call out of order (e.g. run before init) */
+#define CGFD_EHIP (-3)      /* HIP runtime error */
+#define CGFD_ENOMEM (-4)    /* device allocation failed */
+#define CGFD_ERANGE (-5)    /* point outside the valid domain */
+
+typedef struct cgfd_ctx cgfd_ctx;
+
+/* Grid and time stepping (cindx typenew.h:135-141, steph/stept/nt of the
+ * ChildProcs constructor calculate.cu:107-111). */
+typedef struct cgfd_dims {
+    int ni, nj, nk; /* valid cells along x, y, z */
+    double dh;      /* grid step (steph) */
+    double dt;      /* time step (stept) */
+    int nt;         /* number of time steps the run is sized for */
+} cgfd_dims;
+
+/* ChildProcs::ChildProcs (calculate.cu:107-1003): allocate device state on
+ * `device` (HIP ordinal).  precision: CGFD_FP32 or CGFD_FP64. */
+int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out);
+/* ChildProcs::~ChildProcs (calculate.cu:1016-1334) */
+void cgfd_destroy(cgfd_ctx *ctx);
+const char *cgfd_last_error(const cgfd_ctx *ctx);
+
+/* ParH2D for derivF (calculate.cu:2050-2228): 10 arrays xix xiy xiz etax
+ * etay etaz zetax zetay zetaz jac, NX*NY*NZ each, reference layout. */
+int cgfd_set_metrics(cgfd_ctx *ctx, const void *const *drv);
+/* ParH2D for mdparF: P velocity alpha, S velocity beta, density rho. */
+int cgfd_set_media(cgfd_ctx *ctx, const void *alpha, const void *beta, const void *rho);
+/* ParH2D for apara (typenew.h:277-282): nabs[6] = layers on x1 x2 y1 y2 z1 z2
+ * (z2 = top, forced to 0 under the free surface, absorb.cpp:142-148);
+ * tabs[9] = APDx APDy APDz Bx By Bz DBx DBy DBz with lengths NX NY NZ. */
+int cgfd_set_pml(cgfd_ctx *ctx, const int *nabs, const void *const *tabs);
+/* ParH2D for forceF (typedata.h:70-77): n point forces; locs[3n];
+ * vec[3n] = fx fy fz per source (already scaled by the force factor);
+ * stf[n*nstf] sampled every dt/2 (source.cpp:408-416), nstf >= 2*nt+1. */
+int cgfd_set_forces(cgfd_ctx *ctx, int n, const int *locs, const void *vec, const void *stf, int nstf);
+/* ParH2D for momentF (typedata.h:79-86): vec[6n] = mxx myy mzz mxy mxz myz. */
+int cgfd_set_moments(cgfd_ctx *ctx, int n, const int *locs, const void *vec, const void *stf, int nstf);
+/* C2DPointPick (calculate.cu:2678-2746): receivers recording all 9
+ * components of the step-end wavefield every step. */
+int cgfd_set_receivers(cgfd_ctx *ctx, int n, const int *locs);
+/* output_peakvel (PVF, calculate.cu:134): running max |V| on the surface. */
+int cgfd_set_pgv(cgfd_ctx *ctx, int on);
+
+/* VelCoeff (calculate.cu:1336-1353) plus state reset: zero the wavefield,
+ * auxiliary fields, traces and PGV; compute the traction-free closure
+ * matrices; set the step counter to 0. */
+int cgfd_init(cgfd_ctx *ctx);
+
+/* The time loop of simulation.cc:252-717 for one process and one GPU:
+ * advance nsteps whole steps (4 RK stages each, F/B pattern chosen by
+ * step % 4) from the current step.  Asynchronous on the context's stream. */
+int cgfd_run(cgfd_ctx *ctx, int nsteps);
+/* One RKite call (calculate.cu:1355-1599) per stage; the 4th stage of a
+ * step finishes it (state copy, traces, step counter). */
+int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
+int cgfd_sync(cgfd_ctx *ctx);
+int cgfd_current_step(const cgfd_ctx *ctx);
+int cgfd_current_stage(const cgfd_ctx *ctx);
+
+/* State access in the reference layout.  which: 0 = FW (step-start field),
+ * 1 = W (output of the last stage), 2 = tW (RK accumulator).  Ghost cells
+ * read back as 0 and are ignored on write. */
+int cgfd_get_field(cgfd_ctx *ctx, int which, int comp, void *out);
+int cgfd_set_field(cgfd_ctx *ctx, int which, int comp, const void *in);
+/* ADE auxiliary fields in the reference's packed layout
+ * (calculate.cu:325-327): axis 0 [(nabs0+nabs1)][NY][NZ],
+ * axis 1 [(nabs2+nabs3)][NX][NZ], axis 2 [(nabs4+nabs5)][NX][NY];
+ * which as above (0 = FA, 1 = A, 2 = tA). */
+long cgfd_aux_size(const cgfd_ctx *ctx, int axis);
+int cgfd_get_aux(cgfd_ctx *ctx, int which, int axis, int comp, void *out);
+int cgfd_set_aux(cgfd_ctx *ctx, int which, int axis, int comp, const void *in);
+
+/* PWpick results (calculate.cu:2780-2828): [9][nrecv][nt]. */
+int cgfd_get_seismo(cgfd_ctx *ctx, void *out);
+/* SynPV (calculate.cu:1709-1743): [3][NX][NY] (Vx, Vy, Vz maxima). */
+int cgfd_get_pgv(cgfd_ctx *ctx, void *out);
+/* matVx2Vz / matVy2Vz of VelPDcoeff (calculate.cu:3320-3428): [NX][NY][9]. */
+int cgfd_get_cov(cgfd_ctx *ctx, void *covx, void *covy);
+
+/* Measurement hooks (no reference counterpart; the reference only prints
+ * wall-clock ETA, simulation.cc:710-726).  With profiling on, every stage
+ * kernel launch is bracketed by HIP events on the context's stream;
+ * cgfd_profile_read returns the summed milliseconds and launch counts of the
+ * fused stage kernel and of the traction-image kernel. */
+int cgfd_profile(cgfd_ctx *ctx, int on);
+int cgfd_profile_read(cgfd_ctx *ctx, double *ms_stage, long *n_stage, double *ms_timg, long *n_timg);
+/* Launch shape of the fused stage kernel: z-chunks per column sweep and
+ * rows per workgroup (0 keeps the current value). */
+int cgfd_set_tuning(cgfd_ctx *ctx, int zchunks, int rows);
+/* Algorithmic HBM bytes of one RK stage of this configuration (the
+ * 192 B/cell + 144 B/PML-cell-axis model, FP32; doubled for FP64). */
+double cgfd_stage_bytes(const cgfd_ctx *ctx);
+/* The HIP stream (hipStream_t) the context launches on. */
+void *cgfd_stream(const cgfd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (liborc32.so / liborc64.so).
+
+TEST INFRASTRUCTURE ONLY: tests, the smoke check and the CPU-baseline leg of
+bench.py use it as the checker; the product never imports it.  The interface
+mirrors copara-cgfd3d_amd/cgfd3d.py so a scenario can be loaded into either.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+METRICS = ("xix", "xiy", "xiz", "etax", "etay", "etaz", "zetax", "zetay", "zetaz", "jac")
+
+_LIBS = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load(double=False):
+    key = bool(double)
+    if key in _LIBS:
+        return _LIBS[key]
+    path = os.path.join(HERE, "liborc64.so" if double else "liborc32.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    R = ctypes.c_double if double else ctypes.c_float
+    VP, IP = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "orc_create": (VP, [ctypes.c_int, ctypes.c_int, ctypes.c_int, R, R, ctypes.c_int]),
+        "orc_destroy": (None, [VP]),
+        "orc_set_metrics": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
+        "orc_set_media": (ctypes.c_int, [VP, VP, VP, VP]),
+        "orc_set_pml": (ctypes.c_int, [VP, IP, ctypes.POINTER(VP)]),
+        "orc_set_forces": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, VP, ctypes.c_int]),
+        "orc_set_moments": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, VP, ctypes.c_int]),
+        "orc_set_receivers": (ctypes.c_int, [VP, ctypes.c_int, IP]),
+        "orc_set_pgv": (ctypes.c_int, [VP, ctypes.c_int]),
+        "orc_init": (ctypes.c_int, [VP]),
+        "orc_run": (ctypes.c_int, [VP, ctypes.c_int]),
+        "orc_run_stages": (ctypes.c_int, [VP, ctypes.c_int]),
+        "orc_current_step": (ctypes.c_int, [VP]),
+        "orc_current_stage": (ctypes.c_int, [VP]),
+        "orc_get_field": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, VP]),
+        "orc_set_field": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, VP]),
+        "orc_aux_size": (ctypes.c_long, [VP, ctypes.c_int]),
+        "orc_get_aux": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP]),
+        "orc_set_aux": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP]),
+        "orc_get_seismo": (ctypes.c_int, [VP, VP]),
+        "orc_get_pgv": (ctypes.c_int, [VP, VP]),
+        "orc_get_cov": (ctypes.c_int, [VP, VP, VP]),
+        "orc_num_threads": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[key] = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """CPU restatement of the reference time stepping on one whole domain."""
+
+    def __init__(self, ni, nj, nk, dh, dt, nt, double=False):
+        self.lib = load(double)
+        self.dtype = np.float64 if double else np.float32
+        self.ni, self.nj, self.nk, self.nt = ni, nj, nk, nt
+        self.shape = (ni + 6, nj + 6, nk + 6)
+        self.h = self.lib.orc_create(ni, nj, nk, dh, dt, nt)
+        if not self.h:
+            raise OracleError("orc_create failed")
+        self.nrecv = 0
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise OracleError(f"{what} failed ({rc})")
+
+    def set_metrics(self, drv):
+        arrs = [self._arr(drv[k]) for k in METRICS]
+        ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrs])
+        self._check(self.lib.orc_set_metrics(self.h, ptrs), "set_metrics")
+
+    def set_media(self, alpha, beta, rho):
+        a, b, r = (self._arr(v) for v in (alpha, beta, rho))
+        self._check(self.lib.orc_set_media(self.h, _ptr(a), _ptr(b), _ptr(r)), "set_media")
+
+    def set_pml(self, nabs, tabs):
+        order = ("APDx", "APDy", "APDz", "Bx", "By", "Bz", "DBx", "DBy", "DBz")
+        arrs = [self._arr(tabs[k]) for k in order]
+        nab = (ctypes.c_int * 6)(*[int(v) for v in nabs])
+        ptrs = (ctypes.c_void_p * 9)(*[a.ctypes.data for a in arrs])
+        self._check(self.lib.orc_set_pml(self.h, nab, ptrs), "set_pml")
+
+    def set_forces(self, locs, vec, stf):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        vec, stf = self._arr(np.reshape(vec, (n, 3))), self._arr(np.reshape(stf, (n, -1)))
+        self._check(self.lib.orc_set_forces(self.h, n, locs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            _ptr(vec), _ptr(stf), stf.shape[1]), "set_forces")
+
+    def set_moments(self, locs, vec, stf):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        vec, stf = self._arr(np.reshape(vec, (n, 6))), self._arr(np.reshape(stf, (n, -1)))
+        self._check(self.lib.orc_set_moments(self.h, n, locs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             _ptr(vec), _ptr(stf), stf.shape[1]), "set_moments")
+
+    def set_receivers(self, locs):
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        self.nrecv = locs.shape[0]
+        self._check(self.lib.orc_set_receivers(self.h, self.nrecv,
+                                               locs.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "set_receivers")
+
+    def set_pgv(self, on=True):
+        self.lib.orc_set_pgv(self.h, 1 if on else 0)
+
+    def init(self):
+        self._check(self.lib.orc_init(self.h), "init")
+
+    def setup(self, sc):
+        self.set_metrics(sc["metrics"])
+        self.set_media(sc["alpha"], sc["beta"], sc["rho"])
+        self.set_pml(sc["nabs"], sc["pml"])
+        if sc.get("force_locs") is not None and len(sc["force_locs"]):
+            self.set_forces(sc["force_locs"], sc["force_vec"], sc["force_stf"])
+        if sc.get("moment_locs") is not None and len(sc["moment_locs"]):
+            self.set_moments(sc["moment_locs"], sc["moment_vec"], sc["moment_stf"])
+        if sc.get("recv_locs") is not None and len(sc["recv_locs"]):
+            self.set_receivers(sc["recv_locs"])
+        self.set_pgv(sc.get("pgv", True))
+        self.init()
+
+    def run(self, nsteps):
+        self._check(self.lib.orc_run(self.h, int(nsteps)), "run")
+
+    def run_stages(self, n):
+        self._check(self.lib.orc_run_stages(self.h, int(n)), "run_stages")
+
+    @property
+    def step(self):
+        return self.lib.orc_current_step(self.h)
+
+    @property
+    def stage(self):
+        return self.lib.orc_current_stage(self.h)
+
+    def get_field(self, comp, which=0):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self._check(self.lib.orc_get_field(self.h, which, comp, _ptr(out)), "get_field")
+        return out
+
+    def set_field(self, comp, arr, which=0):
+        a = self._arr(arr)
+        self._check(self.lib.orc_set_field(self.h, which, comp, _ptr(a)), "set_field")
+
+    def aux_size(self, axis):
+        return self.lib.orc_aux_size(self.h, axis)
+
+    def get_aux(self, axis, comp, which=0):
+        out = np.empty(max(self.aux_size(axis), 1), dtype=self.dtype)
+        self._check(self.lib.orc_get_aux(self.h, which, axis, comp, _ptr(out)), "get_aux")
+        return out[: self.aux_size(axis)]
+
+    def set_aux(self, axis, comp, arr, which=0):
+        a = self._arr(arr)
+        self._check(self.lib.orc_set_aux(self.h, which, axis, comp, _ptr(a)), "set_aux")
+
+    def get_seismo(self):
+        out = np.zeros((9, self.nrecv, self.nt), dtype=self.dtype)
+        if self.nrecv:
+            self.lib.orc_get_seismo(self.h, _ptr(out))
+        return out
+
+    def get_pgv(self):
+        out = np.empty((3, self.shape[0], self.shape[1]), dtype=self.dtype)
+        self.lib.orc_get_pgv(self.h, _ptr(out))
+        return out
+
+    def get_cov(self):
+        cx = np.empty((self.shape[0], self.shape[1], 9), dtype=self.dtype)
+        cy = np.empty_like(cx)
+        self.lib.orc_get_cov(self.h, _ptr(cx), _ptr(cy))
+        return cx, cy
+
+
+def num_threads():
+    return load(False).orc_num_threads()

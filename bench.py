@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="stage kernel: 0 = LDS-staged planes, 1 = direct loads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -99,6 +100,7 @@ def main():
     sc = build_workload(args.workload, nt)
     log(f"workload {sc['name']} built in {time.perf_counter() - t0:.1f} s")
     sim = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    sim.set_variant(args.variant)
     if args.zchunks or args.rows:
         sim.set_tuning(args.zchunks, args.rows)
     t0 = time.perf_counter()

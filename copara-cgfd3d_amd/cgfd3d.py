@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcgfd3d.so")
+LIB_PATH = os.environ.get("CGFD3D_LIB", os.path.join(HERE, "libcgfd3d.so"))
 
 FP32, FP64 = 0, 1
 COMPONENTS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
@@ -63,6 +63,7 @@ SIGNATURES = {
     "cgfd_profile_read": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
     "cgfd_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "cgfd_set_variant": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_stage_bytes": (ctypes.c_double, [_VP]),
     "cgfd_stream": (_VP, [_VP]),
 }
@@ -260,6 +261,9 @@ class Simulation:
 
     def set_tuning(self, zchunks=0, rows=0):
         self._check(self.lib.cgfd_set_tuning(self.h, int(zchunks), int(rows)), "set_tuning")
+
+    def set_variant(self, variant):
+        self._check(self.lib.cgfd_set_variant(self.h, int(variant)), "set_variant")
 
     def stage_bytes(self):
         return self.lib.cgfd_stage_bytes(self.h)

@@ -70,6 +70,9 @@ struct cgfd_ctx {
     int currT = 0, stage = 0;
     bool inited = false;
     int rows = 4, zchunks = 1;
+    int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
+    char *args_d = nullptr; // the 4 per-stage argument blocks in device memory
+    bool args_dirty = true;
 
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_stage, ev_timg;
@@ -194,7 +197,7 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
     int rc = 0;
     if ((rc = dalloc(c, &c->sets, (size_t)NSETS * NF * g.fs * c->esz)) ||
         (rc = dalloc(c, &c->met, (size_t)NDRV * g.fs * c->esz)) ||
-        (rc = dalloc(c, &c->med, (size_t)3 * g.fs * c->esz)) ||
+        (rc = dalloc(c, &c->med, (size_t)4 * g.fs * c->esz)) ||
         (rc = dalloc(c, &c->pml, (size_t)3 * (g.ni + g.nj + g.nk) * c->esz)) ||
         (rc = dalloc(c, &c->staging, (size_t)c->nref * c->esz)) ||
         (rc = dalloc(c, &c->timg, (size_t)9 * c->tstr * c->esz)) ||
@@ -267,6 +270,7 @@ void cgfd_destroy(cgfd_ctx *c)
     free_sources(c);
     dfree_t(c->rlin);
     dfree(c->seis);
+    dfree(c->args_d);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -315,11 +319,20 @@ int cgfd_set_media(cgfd_ctx *c, const void *alpha, const void *beta, const void 
 {
     if (!c || !alpha || !beta || !rho) return fail(c, CGFD_EINVAL, "null argument");
     HIPCHK(c, hipSetDevice(c->dev));
+    // alpha -> slot 0, beta -> slot 1, rho -> slot 3; then derive
+    // lam2mu, miu, 1/rho in slots 0-2 on the GPU
     const void *src[3] = {alpha, beta, rho};
+    const int slot[3] = {0, 1, 3};
     for (int k = 0; k < 3; k++) {
-        int rc = upload_field(c, src[k], c->med + (size_t)k * c->g.fs * c->esz);
+        int rc = upload_field(c, src[k], c->med + (size_t)slot[k] * c->g.fs * c->esz);
         if (rc) return rc;
     }
+    if (c->prec == CGFD_FP32)
+        launch_media_prep(c->g, (float *)c->med, c->stream);
+    else
+        launch_media_prep(c->g, (double *)c->med, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_media = true;
     c->inited = false;
     return 0;
@@ -435,6 +448,7 @@ int cgfd_set_pgv(cgfd_ctx *c, int on)
 {
     if (!c) return CGFD_EINVAL;
     c->pgv_on = on != 0;
+    c->args_dirty = true;
     return 0;
 }
 
@@ -519,6 +533,7 @@ int cgfd_init(cgfd_ctx *c)
     c->currT = 0;
     c->stage = 0;
     c->inited = true;
+    c->args_dirty = true;
     return 0;
 }
 
@@ -546,33 +561,14 @@ static void fold_events(cgfd_ctx *c)
 
 } // extern "C"
 
-template <typename R> static int run_stage(cgfd_ctx *c)
+static const int kStageArgsBytes = 1024; // room for one StageArgs<double>
+
+template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int k)
 {
-    static const R RK4A[3] = {(R)0.5, (R)0.5, (R)1.0};                          // calculate.cu:12
+    static const R RK4A[3] = {(R)0.5, (R)0.5, (R)1.0};                                      // calculate.cu:12
     static const R RK4B[4] = {(R)(1.0 / 6.0), (R)(1.0 / 3.0), (R)(1.0 / 3.0), (R)(1.0 / 6.0)}; // calculate.cu:13
-    const int k = c->stage;
-    int sx, sz;
-    fb_signs(c->currT, k, &sx, &sz);
-    const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
     const R dt = (R)c->d.dt;
     const int sin_ = stage_in(k), sout = stage_out(k);
-
-    TimgArgs<R> ta{};
-    ta.g = c->g;
-    ta.win = (const R *)c->set_ptr(sin_);
-    ta.met = (const R *)c->met;
-    ta.med = (const R *)c->med;
-    ta.pml = (const R *)c->pml;
-    memcpy(ta.nabs, c->nabs, sizeof(ta.nabs));
-    for (int a = 0; a < 2; a++) {
-        ta.npml[a] = c->npml[a];
-        ta.astr[a] = c->astr[a];
-        ta.ain[a] = (const R *)c->aux_ptr(a, sin_);
-    }
-    ta.timg = (R *)c->timg;
-    ta.tstr = c->tstr;
-    ta.h = c->h;
-
     StageArgs<R> sa{};
     sa.g = c->g;
     sa.win = (const R *)c->set_ptr(sin_);
@@ -602,7 +598,6 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     sa.mstf = (const R *)c->mstf_d;
     sa.nstf_f = c->nstf_f;
     sa.nstf_m = c->nstf_m;
-    sa.tindex = 2 * c->currT + tinc;
     sa.pgv = (k == 3 && c->pgv_on) ? (R *)c->pgv : nullptr;
     sa.h = c->h;
     sa.rh = c->rh;
@@ -610,6 +605,63 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     sa.alpha = k < 3 ? RK4A[k] * dt : (R)0;
     sa.beta = RK4B[k] * dt;
     sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
+    return sa;
+}
+
+// (re)write the four per-stage argument blocks the stage kernels read
+static int upload_args(cgfd_ctx *c)
+{
+    static_assert(sizeof(StageArgs<double>) <= kStageArgsBytes, "argument block too large");
+    std::vector<char> host(4 * kStageArgsBytes, 0);
+    for (int k = 0; k < 4; k++) {
+        if (c->prec == CGFD_FP32) {
+            StageArgs<float> a = make_stage_args<float>(c, k);
+            memcpy(host.data() + k * kStageArgsBytes, &a, sizeof(a));
+        } else {
+            StageArgs<double> a = make_stage_args<double>(c, k);
+            memcpy(host.data() + k * kStageArgsBytes, &a, sizeof(a));
+        }
+    }
+    if (!c->args_d) {
+        int rc = dalloc(c, &c->args_d, host.size());
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->args_d, host.data(), host.size(), hipMemcpyHostToDevice));
+    c->args_dirty = false;
+    return 0;
+}
+
+template <typename R> static int run_stage(cgfd_ctx *c)
+{
+    const int k = c->stage;
+    int sx, sz;
+    fb_signs(c->currT, k, &sx, &sz);
+    const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
+    const int tindex = 2 * c->currT + tinc; // calculate.cu:1404-1421
+    const int sin_ = stage_in(k);
+    if (c->args_dirty) {
+        int rc = upload_args(c);
+        if (rc) return rc;
+    }
+    const StageArgs<R> sa = make_stage_args<R>(c, k);
+    const StageArgs<R> *pa = (const StageArgs<R> *)(c->args_d + k * kStageArgsBytes);
+
+    TimgArgs<R> ta{};
+    ta.g = c->g;
+    ta.win = (const R *)c->set_ptr(sin_);
+    ta.met = (const R *)c->met;
+    ta.med = (const R *)c->med;
+    ta.pml = (const R *)c->pml;
+    memcpy(ta.nabs, c->nabs, sizeof(ta.nabs));
+    for (int a = 0; a < 2; a++) {
+        ta.npml[a] = c->npml[a];
+        ta.astr[a] = c->astr[a];
+        ta.ain[a] = (const R *)c->aux_ptr(a, sin_);
+    }
+    ta.timg = (R *)c->timg;
+    ta.tstr = c->tstr;
+    ta.h = c->h;
 
     // profiling: events bracket each launch on the context's stream
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -622,7 +674,7 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
     }
-    launch_stage<R>(sx, sz, k, sa, c->rows, c->zchunks, c->stream);
+    launch_stage<R>(c->variant, sx, sz, k, sa, pa, tindex, c->rows, c->zchunks, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         c->ev_timg.push_back({ev[0], ev[1]});
@@ -839,6 +891,16 @@ int cgfd_set_tuning(cgfd_ctx *c, int zchunks, int rows)
     if (zchunks < 0 || zchunks > c->g.nk) return fail(c, CGFD_EINVAL, "zchunks out of range");
     if (rows) c->rows = rows;
     if (zchunks) c->zchunks = zchunks;
+    c->args_dirty = true;
+    return 0;
+}
+
+int cgfd_set_variant(cgfd_ctx *c, int variant)
+{
+    if (!c) return CGFD_EINVAL;
+    if (variant < 0 || variant > 2)
+        return fail(c, CGFD_EINVAL, "variant must be 0 (LDS), 1 (direct) or 2 (split velocity/stress)");
+    c->variant = variant;
     return 0;
 }
 

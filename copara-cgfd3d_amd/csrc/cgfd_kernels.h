@@ -14,7 +14,7 @@ template <typename R> struct StageArgs {
     R *wout;      // stage output (FW itself in stage 3)
     R *tw;        // RK accumulator
     const R *met; // 10 metric fields at stride g.fs
-    const R *med; // alpha, beta, rho at stride g.fs
+    const R *med; // lam2mu, miu, 1/rho, rho at stride g.fs
     const R *pml; // APD B DB for x (ni each), then y (nj each), then z (nk each)
     int nabs[6];
     int npml[3];   // packed PML widths per axis
@@ -33,7 +33,6 @@ template <typename R> struct StageArgs {
     const R *fstf;
     const R *mstf;
     int nstf_f, nstf_m;
-    int tindex;
     R *pgv; // 3 slots of [nj][nxp] or null
     double h, rh;
     R hr;
@@ -45,7 +44,7 @@ template <typename R> struct TimgArgs {
     Geom g;
     const R *win;
     const R *met;
-    const R *med;
+    const R *med; // lam2mu, miu, 1/rho, rho
     const R *pml;
     int nabs[6];
     int npml[2];
@@ -56,8 +55,12 @@ template <typename R> struct TimgArgs {
     double h;
 };
 
+// variant 0: LDS-staged xy planes (default); 1: direct loads of the in-plane
+// taps.  `a` is the host copy of the argument block (launch geometry), `pa`
+// the same block in device memory, read by the kernel.
 template <typename R>
-void launch_stage(int sx, int sz, int st, const StageArgs<R> &a, int rows, int zchunks, hipStream_t s);
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex,
+                  int rows, int zchunks, hipStream_t s);
 template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s);
 template <typename R>
 void launch_velpd(const Geom &g, const R *met, const R *med, const R *pml, R *cov, long cstr, hipStream_t s);
@@ -68,6 +71,7 @@ void launch_from_padded(const R *src, R *dst, const Geom &g, hipStream_t s);
 template <typename R>
 void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent, const R *fvec, const R *mvec,
                      R *scoef, int nent, R hr, hipStream_t s);
+template <typename R> void launch_media_prep(const Geom &g, R *med, hipStream_t s);
 template <typename R>
 void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s);
 

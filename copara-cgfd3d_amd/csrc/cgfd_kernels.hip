@@ -30,58 +30,419 @@ namespace cgfd {
 // ---------------------------------------------------------------------------
 // fused RK stage
 // ---------------------------------------------------------------------------
-template <typename R, int SX, int SZ, int ST>
-__global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> a)
+
+// Loads and stores through a uniform field base plus a 32-bit byte offset, so
+// the compiler can use the SGPR-base addressing form (one 32-bit VGPR per
+// address instead of 64-bit pointer arithmetic).  Offsets never exceed one
+// field (< 4 GiB).
+template <typename R> __device__ __forceinline__ R ldg(const R *base, unsigned off)
 {
-    const Geom g = a.g;
+    return *(const R *)((const char *)base + off);
+}
+template <typename R> __device__ __forceinline__ void stg(R *base, unsigned off, R v)
+{
+    *(R *)((char *)base + off) = v;
+}
+
+// member-wise copy of the geometry out of a constant-address-space block
+template <typename AP> __device__ __forceinline__ Geom geom_of(AP a)
+{
+    Geom g;
+    g.ni = a->g.ni;
+    g.nj = a->g.nj;
+    g.nk = a->g.nk;
+    g.nxp = a->g.nxp;
+    g.nyp = a->g.nyp;
+    g.plane = a->g.plane;
+    g.fs = a->g.fs;
+    g.base = a->g.base;
+    return g;
+}
+
+// per-(x,y)-column constants of the PML and source bookkeeping
+template <typename R> struct Column {
+    R APDx, Bx, DBx, APDy, By, DBy;
+    bool inx, iny;
+    int ix, iy;
+    unsigned cxy; // element offset of (x,y) inside one [nj][nxp] map
+    int col;      // y*ni + x, the source-table column id
+};
+
+template <typename R, typename AP> __device__ __forceinline__ Column<R> column_of(AP a, int x, int y)
+{
+    const Geom g = geom_of(a);
+    Column<R> k;
+    const R *pmx = a->pml;
+    const R *pmy = a->pml + 3 * g.ni;
+    k.APDx = pmx[x];
+    k.Bx = pmx[g.ni + x];
+    k.DBx = pmx[2 * g.ni + x];
+    k.APDy = pmy[y];
+    k.By = pmy[g.nj + y];
+    k.DBy = pmy[2 * g.nj + y];
+    k.inx = x < a->nabs[0] || x >= g.ni - a->nabs[1];
+    k.ix = x < a->nabs[0] ? x : x - (g.ni - a->nabs[1]) + a->nabs[0];
+    k.iny = y < a->nabs[2] || y >= g.nj - a->nabs[3];
+    k.iy = y < a->nabs[2] ? y : y - (g.nj - a->nabs[3]) + a->nabs[2];
+    k.cxy = (unsigned)(y * g.nxp + x);
+    k.col = y * g.ni + x;
+    return k;
+}
+
+// first source entry of this column at or above z0 (entries are sorted by
+// column, then z)
+template <typename AP> __device__ __forceinline__ int first_source(AP a, int col, int z0)
+{
+    int e = a->col_first ? a->col_first[col] : -1;
+    if (e >= 0)
+        while (a->sent[e].w == col && a->sent[e].x < z0) ++e;
+    return e;
+}
+
+// Per-cell inputs that do not depend on the stencil: the media, the nine
+// metrics and the RK operands (FW in stages 1-2, tW in stages 1-3).  They are
+// fetched at the top of each plane so their latency overlaps the stencil
+// arithmetic instead of stalling the RK update.
+template <typename R> struct CellIn {
+    R lam2mu, miu, rhoinv;
+    R m[9];
+    R fw[NF], tw[NF];
+};
+
+template <typename R, int ST, typename AP>
+__device__ __forceinline__ void load_cell(AP a, unsigned pb, long fs, CellIn<R> &in)
+{
+    in.lam2mu = ldg(a->med, pb);
+    in.miu = ldg(a->med + fs, pb);
+    in.rhoinv = ldg(a->med + 2 * fs, pb);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) in.m[m] = ldg(a->met + m * fs, pb);
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        if (ST == 1 || ST == 2) in.fw[c] = ldg(a->fw + c * fs, pb);
+        if (ST >= 1) in.tw[c] = ldg((const R *)a->tw + c * fs, pb);
+    }
+}
+
+// Everything of one cell after its 27 derivatives are known: the free-surface
+// velocity closure, CalWaveCL, the ADE CFS-PML rates with their RK update,
+// the traction-image override, the point sources and the RK update of the
+// wavefield (plus the PGV maximum in the last stage).
+//   w: z window of the stage input (slot CI is the cell itself)
+//   pb: byte offset of the cell inside a field
+template <typename R, int SX, int SZ, int ST, typename AP>
+__device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const CellIn<R> &in, R (&Dx)[NF], R (&Dy)[NF],
+                                            R (&Dz)[NF], const R (&w)[NF][5], int x, int y, int z, unsigned pb,
+                                            int &e, long fs, int tindex)
+{
+    const Geom g = geom_of(a);
+    const double h = a->h;
+    constexpr int CI = SZ > 0 ? 1 : 3;
+
+    // velocity zeta-derivatives of the top three rows (CondFreeVLOW,
+    // calculate.cu:4736-4769): closure on the surface, 2-point one row
+    // below, 2-4 MacCormack two rows below
+    const int top = z - (g.nk - 3);
+    R cx[9], cy[9];
+    if (top == 2) {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+            cx[m] = a->cov[m * a->cstr + k.cxy];
+            cy[m] = a->cov[(9 + m) * a->cstr + k.cxy];
+        }
+        Dz[VX] = cx[0] * Dx[VX] + cx[1] * Dx[VY] + cx[2] * Dx[VZ] + cy[0] * Dy[VX] + cy[1] * Dy[VY] + cy[2] * Dy[VZ];
+        Dz[VY] = cx[3] * Dx[VX] + cx[4] * Dx[VY] + cx[5] * Dx[VZ] + cy[3] * Dy[VX] + cy[4] * Dy[VY] + cy[5] * Dy[VZ];
+        Dz[VZ] = cx[6] * Dx[VX] + cx[7] * Dx[VY] + cx[8] * Dx[VZ] + cy[6] * Dy[VX] + cy[7] * Dy[VY] + cy[8] * Dy[VZ];
+    } else if (top == 1) {
+#pragma unroll
+        for (int c = VX; c <= VZ; ++c)
+            Dz[c] = SZ > 0 ? (w[c][2] - w[c][1]) / a->hr : (w[c][3] - w[c][2]) / a->hr;
+    } else if (top == 0) {
+#pragma unroll
+        for (int c = VX; c <= VZ; ++c) {
+            double s = SZ > 0 ? M24A0 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A2 * (double)w[c][3]
+                              : M24A2 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A0 * (double)w[c][3];
+            Dz[c] = (R)(SZ > 0 ? s / h : s / (-h));
+        }
+    }
+
+    // ---- CalWaveCL: contraction with metrics, Hooke, momentum -------------
+    // media are stored as lam2mu = rho*alpha*alpha, miu = rho*beta*beta and
+    // 1/rho, each rounded exactly as calculate.cu:4855-4858 computes them
+    const R lam2mu = in.lam2mu, miu = in.miu, rho = in.rhoinv;
+    const R lambda = lam2mu - (R)2 * miu; // 2*miu is exact: one rounding, as the reference's double expression
+    const R xix = in.m[XIX], xiy = in.m[XIY], xiz = in.m[XIZ];
+    const R etx = in.m[ETX], ety = in.m[ETY], etz = in.m[ETZ];
+    const R ztx = in.m[ZTX], zty = in.m[ZTY], ztz = in.m[ZTZ];
+
+    const R DxiVx = (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ]) * rho;
+    const R DetVx = (etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ]) * rho;
+    const R DztVx = (ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]) * rho;
+    const R DxiVy = (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ]) * rho;
+    const R DetVy = (etx * Dy[TXY] + ety * Dy[TYY] + etz * Dy[TYZ]) * rho;
+    const R DztVy = (ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]) * rho;
+    const R DxiVz = (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ]) * rho;
+    const R DetVz = (etx * Dy[TXZ] + ety * Dy[TYZ] + etz * Dy[TZZ]) * rho;
+    const R DztVz = (ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]) * rho;
+
+    const R DxiTxx = lam2mu * xix * Dx[VX] + lambda * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
+    const R DetTxx = lam2mu * etx * Dy[VX] + lambda * ety * Dy[VY] + lambda * etz * Dy[VZ];
+    const R DztTxx = lam2mu * ztx * Dz[VX] + lambda * zty * Dz[VY] + lambda * ztz * Dz[VZ];
+    const R DxiTyy = lambda * xix * Dx[VX] + lam2mu * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
+    const R DetTyy = lambda * etx * Dy[VX] + lam2mu * ety * Dy[VY] + lambda * etz * Dy[VZ];
+    const R DztTyy = lambda * ztx * Dz[VX] + lam2mu * zty * Dz[VY] + lambda * ztz * Dz[VZ];
+    const R DxiTzz = lambda * xix * Dx[VX] + lambda * xiy * Dx[VY] + lam2mu * xiz * Dx[VZ];
+    const R DetTzz = lambda * etx * Dy[VX] + lambda * ety * Dy[VY] + lam2mu * etz * Dy[VZ];
+    const R DztTzz = lambda * ztx * Dz[VX] + lambda * zty * Dz[VY] + lam2mu * ztz * Dz[VZ];
+    const R DxiTxy = (xiy * Dx[VX] + xix * Dx[VY]) * miu;
+    const R DetTxy = (ety * Dy[VX] + etx * Dy[VY]) * miu;
+    const R DztTxy = (zty * Dz[VX] + ztx * Dz[VY]) * miu;
+    const R DxiTxz = (xiz * Dx[VX] + xix * Dx[VZ]) * miu;
+    const R DetTxz = (etz * Dy[VX] + etx * Dy[VZ]) * miu;
+    const R DztTxz = (ztz * Dz[VX] + ztx * Dz[VZ]) * miu;
+    const R DxiTyz = (xiz * Dx[VY] + xiy * Dx[VZ]) * miu;
+    const R DetTyz = (etz * Dy[VY] + ety * Dy[VZ]) * miu;
+    const R DztTyz = (ztz * Dz[VY] + zty * Dz[VZ]) * miu;
+
+    const R *pmz = a->pml + 3 * g.ni + 3 * g.nj;
+    const R APDz = pmz[z], Bz = pmz[g.nk + z], DBz = pmz[2 * g.nk + z];
+    const R Bx = k.Bx, By = k.By;
+    R hv[NF];
+    if (Bx == (R)1 && By == (R)1 && Bz == (R)1) {
+        // x/1 == x exactly: the interior skips the nine divisions
+        hv[TXX] = DxiTxx + DetTxx + DztTxx;
+        hv[TYY] = DxiTyy + DetTyy + DztTyy;
+        hv[TZZ] = DxiTzz + DetTzz + DztTzz;
+        hv[TXY] = DxiTxy + DetTxy + DztTxy;
+        hv[TXZ] = DxiTxz + DetTxz + DztTxz;
+        hv[TYZ] = DxiTyz + DetTyz + DztTyz;
+        hv[VX] = DxiVx + DetVx + DztVx;
+        hv[VY] = DxiVy + DetVy + DztVy;
+        hv[VZ] = DxiVz + DetVz + DztVz;
+    } else {
+        hv[TXX] = DxiTxx / Bx + DetTxx / By + DztTxx / Bz;
+        hv[TYY] = DxiTyy / Bx + DetTyy / By + DztTyy / Bz;
+        hv[TZZ] = DxiTzz / Bx + DetTzz / By + DztTzz / Bz;
+        hv[TXY] = DxiTxy / Bx + DetTxy / By + DztTxy / Bz;
+        hv[TXZ] = DxiTxz / Bx + DetTxz / By + DztTxz / Bz;
+        hv[TYZ] = DxiTyz / Bx + DetTyz / By + DztTyz / Bz;
+        hv[VX] = DxiVx / Bx + DetVx / By + DztVx / Bz;
+        hv[VY] = DxiVy / Bx + DetVy / By + DztVy / Bz;
+        hv[VZ] = DxiVz / Bx + DetVz / By + DztVz / Bz;
+    }
+
+    const unsigned tcell = (unsigned)(top > 0 ? top : 0) * (unsigned)(g.nj * g.nxp) + k.cxy; // top-row slot
+
+    // ---- ADE CFS-PML (calculate.cu:4993-5107) + auxiliary RK update --------
+    // The rates of one axis feed that axis' RK update straight away; the
+    // traction image replaces the velocity rates of the top rows first.
+#define CGFD_AUX_UPDATE(AX, HA, AIN)                                                                      \
+    {                                                                                                     \
+        const long ap = aidx;                                                                             \
+        const long as = a->astr[AX];                                                                       \
+        _Pragma("unroll") for (int c = 0; c < NF; ++c)                                                    \
+        {                                                                                                 \
+            const R hh = HA[c];                                                                           \
+            if (ST == 0) {                                                                                \
+                a->aout[AX][c * as + ap] = AIN[c] + a->alpha * hh;                                          \
+                a->at[AX][c * as + ap] = AIN[c] + a->beta * hh;                                             \
+            } else if (ST < 3) {                                                                          \
+                a->aout[AX][c * as + ap] = a->af[AX][c * as + ap] + a->alpha * hh;                           \
+                a->at[AX][c * as + ap] = a->at[AX][c * as + ap] + a->beta * hh;                              \
+            } else {                                                                                      \
+                a->aout[AX][c * as + ap] = a->at[AX][c * as + ap] + a->beta * hh;                            \
+            }                                                                                             \
+        }                                                                                                 \
+    }
+
+    R DzVx1 = 0, DzVx2 = 0, DzVy1 = 0, DzVy2 = 0, DzVz1 = 0, DzVz2 = 0;
+    if (top == 2 && (k.inx || k.iny)) {
+        // surface contributions routed into the x/y auxiliary stress rates
+        DzVx1 = cx[0] * Dx[VX] + cx[1] * Dx[VY] + cx[2] * Dx[VZ];
+        DzVx2 = cy[0] * Dy[VX] + cy[1] * Dy[VY] + cy[2] * Dy[VZ];
+        DzVy1 = cx[3] * Dx[VX] + cx[4] * Dx[VY] + cx[5] * Dx[VZ];
+        DzVy2 = cy[3] * Dy[VX] + cy[4] * Dy[VY] + cy[5] * Dy[VZ];
+        DzVz1 = cx[6] * Dx[VX] + cx[7] * Dx[VY] + cx[8] * Dx[VZ];
+        DzVz2 = cy[6] * Dy[VX] + cy[7] * Dy[VY] + cy[8] * Dy[VZ];
+    }
+    if (k.inx) {
+        const R APDx = k.APDx, DBx = k.DBx;
+        const long aidx = ((long)z * g.nj + y) * a->npml[0] + k.ix;
+        R ai[NF], ha[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) ai[c] = a->ain[0][c * a->astr[0] + aidx];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bx;
+        ha[VX] = DxiVx * DBx - APDx * ai[VX];
+        ha[VY] = DxiVy * DBx - APDx * ai[VY];
+        ha[VZ] = DxiVz * DBx - APDx * ai[VZ];
+        ha[TXX] = DxiTxx * DBx - APDx * ai[TXX];
+        ha[TYY] = DxiTyy * DBx - APDx * ai[TYY];
+        ha[TZZ] = DxiTzz * DBx - APDx * ai[TZZ];
+        ha[TXY] = DxiTxy * DBx - APDx * ai[TXY];
+        ha[TXZ] = DxiTxz * DBx - APDx * ai[TXZ];
+        ha[TYZ] = DxiTyz * DBx - APDx * ai[TYZ];
+        if (top == 2) {
+            ha[TXX] += DBx * Bx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
+            ha[TYY] += DBx * Bx * (lambda * ztx * DzVx1 + lam2mu * zty * DzVy1 + lambda * ztz * DzVz1);
+            ha[TZZ] += DBx * Bx * (lambda * ztx * DzVx1 + lambda * zty * DzVy1 + lam2mu * ztz * DzVz1);
+            ha[TXY] += DBx * Bx * (zty * DzVx1 + ztx * DzVy1) * miu;
+            ha[TXZ] += DBx * Bx * (ztz * DzVx1 + ztx * DzVz1) * miu;
+            ha[TYZ] += DBx * Bx * (ztz * DzVy1 + zty * DzVz1) * miu;
+        }
+        if (top >= 0) {
+#pragma unroll
+            for (int c = VX; c <= VZ; ++c) ha[c] = a->timg[(3 + c) * a->tstr + tcell];
+        }
+        CGFD_AUX_UPDATE(0, ha, ai)
+    }
+    if (k.iny) {
+        const R APDy = k.APDy, DBy = k.DBy;
+        const long aidx = ((long)z * a->npml[1] + k.iy) * g.nxp + x;
+        R ai[NF], ha[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) ai[c] = a->ain[1][c * a->astr[1] + aidx];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / By;
+        ha[VX] = DetVx * DBy - APDy * ai[VX];
+        ha[VY] = DetVy * DBy - APDy * ai[VY];
+        ha[VZ] = DetVz * DBy - APDy * ai[VZ];
+        ha[TXX] = DetTxx * DBy - APDy * ai[TXX];
+        ha[TYY] = DetTyy * DBy - APDy * ai[TYY];
+        ha[TZZ] = DetTzz * DBy - APDy * ai[TZZ];
+        ha[TXY] = DetTxy * DBy - APDy * ai[TXY];
+        ha[TXZ] = DetTxz * DBy - APDy * ai[TXZ];
+        ha[TYZ] = DetTyz * DBy - APDy * ai[TYZ];
+        if (top == 2) {
+            ha[TXX] += DBy * By * (lam2mu * ztx * DzVx2 + lambda * zty * DzVy2 + lambda * ztz * DzVz2);
+            ha[TYY] += DBy * By * (lambda * ztx * DzVx2 + lam2mu * zty * DzVy2 + lambda * ztz * DzVz2);
+            ha[TZZ] += DBy * By * (lambda * ztx * DzVx2 + lambda * zty * DzVy2 + lam2mu * ztz * DzVz2);
+            ha[TXY] += DBy * By * (zty * DzVx2 + ztx * DzVy2) * miu;
+            ha[TXZ] += DBy * By * (ztz * DzVx2 + ztx * DzVz2) * miu;
+            ha[TYZ] += DBy * By * (ztz * DzVy2 + zty * DzVz2) * miu;
+        }
+        if (top >= 0) {
+#pragma unroll
+            for (int c = VX; c <= VZ; ++c) ha[c] = a->timg[(6 + c) * a->tstr + tcell];
+        }
+        CGFD_AUX_UPDATE(1, ha, ai)
+    }
+    if (z < a->nabs[4] || z >= g.nk - a->nabs[5]) {
+        const int iz = z < a->nabs[4] ? z : z - (g.nk - a->nabs[5]) + a->nabs[4];
+        const long aidx = ((long)iz * g.nj + y) * g.nxp + x;
+        R ai[NF], ha[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) ai[c] = a->ain[2][c * a->astr[2] + aidx];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bz;
+        ha[VX] = DztVx * DBz - APDz * ai[VX];
+        ha[VY] = DztVy * DBz - APDz * ai[VY];
+        ha[VZ] = DztVz * DBz - APDz * ai[VZ];
+        ha[TXX] = DztTxx * DBz - APDz * ai[TXX];
+        ha[TYY] = DztTyy * DBz - APDz * ai[TYY];
+        ha[TZZ] = DztTzz * DBz - APDz * ai[TZZ];
+        ha[TXY] = DztTxy * DBz - APDz * ai[TXY];
+        ha[TXZ] = DztTxz * DBz - APDz * ai[TXZ];
+        ha[TYZ] = DztTyz * DBz - APDz * ai[TYZ];
+        CGFD_AUX_UPDATE(2, ha, ai)
+    }
+#undef CGFD_AUX_UPDATE
+
+    // ---- CalTIMG override of the velocity rates on the top three rows -----
+    if (top >= 0) {
+#pragma unroll
+        for (int c = VX; c <= VZ; ++c) hv[c] = a->timg[c * a->tstr + tcell];
+    }
+
+    // ---- LoadForce then LoadMoment, in source order ------------------------
+    if (e >= 0) {
+        int4 en = a->sent[e];
+        while (en.w == k.col && en.x == z) {
+            const R *cf = a->scoef + (long)e * 8;
+            if (en.y == 0) {
+                const R stf = a->fstf[(long)en.z * a->nstf_f + tindex];
+                const R A = cf[0];
+                hv[VX] += stf * cf[1] * A;
+                hv[VY] += stf * cf[2] * A;
+                hv[VZ] += stf * cf[3] * A;
+            } else {
+                const R stf = a->mstf[(long)en.z * a->nstf_m + tindex];
+                R A = (R)1;
+                A = stf * A / cf[0];
+                hv[TXX] -= cf[1] * A;
+                hv[TYY] -= cf[2] * A;
+                hv[TZZ] -= cf[3] * A;
+                hv[TXY] -= cf[4] * A;
+                hv[TXZ] -= cf[5] * A;
+                hv[TYZ] -= cf[6] * A;
+            }
+            en = a->sent[++e];
+        }
+    }
+
+    // ---- RK4 update (Iteration*, calculate.cu:6330-6852) -------------------
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        if (ST == 0) {
+            const R f0 = w[c][CI]; // the stage input is FW itself
+            stg(a->wout + c * fs, pb, f0 + a->alpha * hv[c]);
+            stg(a->tw + c * fs, pb, f0 + a->beta * hv[c]);
+        } else if (ST < 3) {
+            stg(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c]);
+            stg(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c]);
+        } else {
+            const R wn = in.tw[c] + a->beta * hv[c];
+            stg(a->wout + c * fs, pb, wn);
+            if (c < 3 && top == 2 && a->pgv) {
+                R *pv = a->pgv + c * a->cstr + k.cxy; // PGV maps share the [nj][nxp] slot size of cov
+                *pv = rmax(rabs(wn), rabs(*pv));
+            }
+        }
+    }
+}
+
+// Variant "direct": the x and y stencil taps are read straight from memory
+// (vector L1/L2); one lane per (x,y) column sweeps its z chunk with the z taps
+// in a 5-deep register window.
+template <typename R, int SX, int SZ, int ST>
+__global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *pa, int tindex)
+{
+    typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
+    const AP a = (AP)pa;
+    const Geom g = geom_of(a);
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= g.ni || y >= g.nj) return;
-    const int z0 = blockIdx.z * a.zlen;
-    const int z1 = min(g.nk, z0 + a.zlen);
+    const int z0 = blockIdx.z * a->zlen;
+    const int z1 = min(g.nk, z0 + a->zlen);
     if (z0 >= z1) return;
 
     const long fs = g.fs;
     const long row = g.nxp;
     const long pl = g.plane;
-    const double h = a.h, rh = a.rh;
+    const double h = a->h, rh = a->rh;
     constexpr int ZOFF = SZ > 0 ? -1 : -3; // window slot 0 sits at z+ZOFF
     constexpr int CI = SZ > 0 ? 1 : 3;     // window slot of the cell itself
 
-    // ---- per-column constants --------------------------------------------
-    const R *pmx = a.pml;
-    const R *pmy = a.pml + 3 * g.ni;
-    const R *pmz = pmy + 3 * g.nj;
-    const R APDx = pmx[x], Bx = pmx[g.ni + x], DBx = pmx[2 * g.ni + x];
-    const R APDy = pmy[y], By = pmy[g.nj + y], DBy = pmy[2 * g.nj + y];
-    const bool inx = x < a.nabs[0] || x >= g.ni - a.nabs[1];
-    const int ix = x < a.nabs[0] ? x : x - (g.ni - a.nabs[1]) + a.nabs[0];
-    const bool iny = y < a.nabs[2] || y >= g.nj - a.nabs[3];
-    const int iy = y < a.nabs[2] ? y : y - (g.nj - a.nabs[3]) + a.nabs[2];
-    const long cxy = (long)y * row + x; // (x,y) offset inside one plane-sized map
+    const Column<R> k = column_of<R>(a, x, y);
+    int e = first_source(a, k.col, z0);
 
-    const int col = y * g.ni + x;
-    int e = a.col_first ? a.col_first[col] : -1;
-    if (e >= 0)
-        while (a.sent[e].w == col && a.sent[e].x < z0) ++e;
-
-    // ---- z register window -------------------------------------------------
     long p = gidx(g, x, y, z0);
     R w[NF][5];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int c = 0; c < NF; ++c) w[c][j] = a.win[c * fs + p + (ZOFF + j) * pl];
+        for (int c = 0; c < NF; ++c) w[c][j] = a->win[c * fs + p + (ZOFF + j) * pl];
 
     for (int z = z0; z < z1; ++z, p += pl) {
 #pragma unroll
-        for (int c = 0; c < NF; ++c) w[c][4] = a.win[c * fs + p + (ZOFF + 4) * pl];
+        for (int c = 0; c < NF; ++c) w[c][4] = a->win[c * fs + p + (ZOFF + 4) * pl];
 
         // ---- CalDiffCL: one-sided derivatives along xi, eta, zeta -------------
         R Dx[NF], Dy[NF], Dz[NF];
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
-            const R *f = a.win + c * fs + p;
+            const R *f = a->win + c * fs + p;
             const R ctr = w[c][CI];
             if (SX > 0) {
                 Dx[c] = dfd<R, 1>(f[-1], ctr, f[1], f[2], f[3], h, rh);
@@ -92,279 +453,561 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> a)
             }
             Dz[c] = dfd<R, SZ>(w[c][0], w[c][1], w[c][2], w[c][3], w[c][4], h, rh);
         }
+        CellIn<R> cin;
+        load_cell<R, ST>(a, (unsigned)(p * sizeof(R)), fs, cin);
+        finish_cell<R, SX, SZ, ST>(a, k, cin, Dx, Dy, Dz, w, x, y, z, (unsigned)(p * sizeof(R)), e, fs, tindex);
 
-        // velocity zeta-derivatives of the top three rows (CondFreeVLOW,
-        // calculate.cu:4736-4769): closure on the surface, 2-point one row
-        // below, 2-4 MacCormack two rows below
-        const int top = z - (g.nk - 3);
-        R cx[9], cy[9];
-        if (top == 2) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                cx[k] = a.cov[k * a.cstr + cxy];
-                cy[k] = a.cov[(9 + k) * a.cstr + cxy];
-            }
-            Dz[VX] = cx[0] * Dx[VX] + cx[1] * Dx[VY] + cx[2] * Dx[VZ] + cy[0] * Dy[VX] + cy[1] * Dy[VY] +
-                     cy[2] * Dy[VZ];
-            Dz[VY] = cx[3] * Dx[VX] + cx[4] * Dx[VY] + cx[5] * Dx[VZ] + cy[3] * Dy[VX] + cy[4] * Dy[VY] +
-                     cy[5] * Dy[VZ];
-            Dz[VZ] = cx[6] * Dx[VX] + cx[7] * Dx[VY] + cx[8] * Dx[VZ] + cy[6] * Dy[VX] + cy[7] * Dy[VY] +
-                     cy[8] * Dy[VZ];
-        } else if (top == 1) {
-#pragma unroll
-            for (int c = VX; c <= VZ; ++c)
-                Dz[c] = SZ > 0 ? (w[c][2] - w[c][1]) / a.hr : (w[c][3] - w[c][2]) / a.hr;
-        } else if (top == 0) {
-#pragma unroll
-            for (int c = VX; c <= VZ; ++c) {
-                double s = SZ > 0 ? M24A0 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A2 * (double)w[c][3]
-                                  : M24A2 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A0 * (double)w[c][3];
-                Dz[c] = (R)(SZ > 0 ? s / h : s / (-h));
-            }
-        }
-
-        // ---- CalWaveCL: contraction with metrics, Hooke, momentum -------------
-        const R *mt = a.met + p;
-        const R *md = a.med + p;
-        R rho = md[2 * fs];
-        const R miu = rho * md[fs] * md[fs];
-        const R lam2mu = rho * md[0] * md[0];
-        const R lambda = lam2mu - (R)2 * miu; // exact: 2*miu is exact, single rounding
-        rho = (R)1 / rho;
-        const R xix = mt[0], xiy = mt[fs], xiz = mt[2 * fs];
-        const R etx = mt[3 * fs], ety = mt[4 * fs], etz = mt[5 * fs];
-        const R ztx = mt[6 * fs], zty = mt[7 * fs], ztz = mt[8 * fs];
-
-        const R DxiVx = (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ]) * rho;
-        const R DetVx = (etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ]) * rho;
-        const R DztVx = (ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]) * rho;
-        const R DxiVy = (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ]) * rho;
-        const R DetVy = (etx * Dy[TXY] + ety * Dy[TYY] + etz * Dy[TYZ]) * rho;
-        const R DztVy = (ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]) * rho;
-        const R DxiVz = (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ]) * rho;
-        const R DetVz = (etx * Dy[TXZ] + ety * Dy[TYZ] + etz * Dy[TZZ]) * rho;
-        const R DztVz = (ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]) * rho;
-
-        const R DxiTxx = lam2mu * xix * Dx[VX] + lambda * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
-        const R DetTxx = lam2mu * etx * Dy[VX] + lambda * ety * Dy[VY] + lambda * etz * Dy[VZ];
-        const R DztTxx = lam2mu * ztx * Dz[VX] + lambda * zty * Dz[VY] + lambda * ztz * Dz[VZ];
-        const R DxiTyy = lambda * xix * Dx[VX] + lam2mu * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
-        const R DetTyy = lambda * etx * Dy[VX] + lam2mu * ety * Dy[VY] + lambda * etz * Dy[VZ];
-        const R DztTyy = lambda * ztx * Dz[VX] + lam2mu * zty * Dz[VY] + lambda * ztz * Dz[VZ];
-        const R DxiTzz = lambda * xix * Dx[VX] + lambda * xiy * Dx[VY] + lam2mu * xiz * Dx[VZ];
-        const R DetTzz = lambda * etx * Dy[VX] + lambda * ety * Dy[VY] + lam2mu * etz * Dy[VZ];
-        const R DztTzz = lambda * ztx * Dz[VX] + lambda * zty * Dz[VY] + lam2mu * ztz * Dz[VZ];
-        const R DxiTxy = (xiy * Dx[VX] + xix * Dx[VY]) * miu;
-        const R DetTxy = (ety * Dy[VX] + etx * Dy[VY]) * miu;
-        const R DztTxy = (zty * Dz[VX] + ztx * Dz[VY]) * miu;
-        const R DxiTxz = (xiz * Dx[VX] + xix * Dx[VZ]) * miu;
-        const R DetTxz = (etz * Dy[VX] + etx * Dy[VZ]) * miu;
-        const R DztTxz = (ztz * Dz[VX] + ztx * Dz[VZ]) * miu;
-        const R DxiTyz = (xiz * Dx[VY] + xiy * Dx[VZ]) * miu;
-        const R DetTyz = (etz * Dy[VY] + ety * Dy[VZ]) * miu;
-        const R DztTyz = (ztz * Dz[VY] + zty * Dz[VZ]) * miu;
-
-        const R APDz = pmz[z], Bz = pmz[g.nk + z], DBz = pmz[2 * g.nk + z];
-        R hv[NF];
-        if (Bx == (R)1 && By == (R)1 && Bz == (R)1) {
-            // x/1 == x exactly: the interior skips the nine divisions
-            hv[TXX] = DxiTxx + DetTxx + DztTxx;
-            hv[TYY] = DxiTyy + DetTyy + DztTyy;
-            hv[TZZ] = DxiTzz + DetTzz + DztTzz;
-            hv[TXY] = DxiTxy + DetTxy + DztTxy;
-            hv[TXZ] = DxiTxz + DetTxz + DztTxz;
-            hv[TYZ] = DxiTyz + DetTyz + DztTyz;
-            hv[VX] = DxiVx + DetVx + DztVx;
-            hv[VY] = DxiVy + DetVy + DztVy;
-            hv[VZ] = DxiVz + DetVz + DztVz;
-        } else {
-            hv[TXX] = DxiTxx / Bx + DetTxx / By + DztTxx / Bz;
-            hv[TYY] = DxiTyy / Bx + DetTyy / By + DztTyy / Bz;
-            hv[TZZ] = DxiTzz / Bx + DetTzz / By + DztTzz / Bz;
-            hv[TXY] = DxiTxy / Bx + DetTxy / By + DztTxy / Bz;
-            hv[TXZ] = DxiTxz / Bx + DetTxz / By + DztTxz / Bz;
-            hv[TYZ] = DxiTyz / Bx + DetTyz / By + DztTyz / Bz;
-            hv[VX] = DxiVx / Bx + DetVx / By + DztVx / Bz;
-            hv[VY] = DxiVy / Bx + DetVy / By + DztVy / Bz;
-            hv[VZ] = DxiVz / Bx + DetVz / By + DztVz / Bz;
-        }
-
-        // surface contributions routed into the x/y auxiliary stress rates
-        R DzVx1 = 0, DzVx2 = 0, DzVy1 = 0, DzVy2 = 0, DzVz1 = 0, DzVz2 = 0;
-        if (top == 2 && (inx || iny)) {
-            DzVx1 = cx[0] * Dx[VX] + cx[1] * Dx[VY] + cx[2] * Dx[VZ];
-            DzVx2 = cy[0] * Dy[VX] + cy[1] * Dy[VY] + cy[2] * Dy[VZ];
-            DzVy1 = cx[3] * Dx[VX] + cx[4] * Dx[VY] + cx[5] * Dx[VZ];
-            DzVy2 = cy[3] * Dy[VX] + cy[4] * Dy[VY] + cy[5] * Dy[VZ];
-            DzVz1 = cx[6] * Dx[VX] + cx[7] * Dx[VY] + cx[8] * Dx[VZ];
-            DzVz2 = cy[6] * Dy[VX] + cy[7] * Dy[VY] + cy[8] * Dy[VZ];
-        }
-        const long tcell = (long)(top > 0 ? top : 0) * g.nj * row + cxy; // top-row slot offset
-
-        // ---- ADE CFS-PML (calculate.cu:4993-5107) + auxiliary RK update --------
-        // The rates of one axis feed that axis' RK update straight away; the
-        // traction image replaces the velocity rates of the top rows first.
-#define CGFD_AUX_UPDATE(AX, HA, AIN)                                                                      \
-    {                                                                                                     \
-        const long ap = aidx;                                                                             \
-        const long as = a.astr[AX];                                                                       \
-        _Pragma("unroll") for (int c = 0; c < NF; ++c)                                                    \
-        {                                                                                                 \
-            const R hh = HA[c];                                                                           \
-            if (ST == 0) {                                                                                \
-                a.aout[AX][c * as + ap] = AIN[c] + a.alpha * hh;                                          \
-                a.at[AX][c * as + ap] = AIN[c] + a.beta * hh;                                             \
-            } else if (ST < 3) {                                                                          \
-                a.aout[AX][c * as + ap] = a.af[AX][c * as + ap] + a.alpha * hh;                           \
-                a.at[AX][c * as + ap] = a.at[AX][c * as + ap] + a.beta * hh;                              \
-            } else {                                                                                      \
-                a.aout[AX][c * as + ap] = a.at[AX][c * as + ap] + a.beta * hh;                            \
-            }                                                                                             \
-        }                                                                                                 \
-    }
-
-        if (inx) {
-            const long aidx = ((long)z * g.nj + y) * a.npml[0] + ix;
-            R ai[NF], ha[NF];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) ai[c] = a.ain[0][c * a.astr[0] + aidx];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bx;
-            ha[VX] = DxiVx * DBx - APDx * ai[VX];
-            ha[VY] = DxiVy * DBx - APDx * ai[VY];
-            ha[VZ] = DxiVz * DBx - APDx * ai[VZ];
-            ha[TXX] = DxiTxx * DBx - APDx * ai[TXX];
-            ha[TYY] = DxiTyy * DBx - APDx * ai[TYY];
-            ha[TZZ] = DxiTzz * DBx - APDx * ai[TZZ];
-            ha[TXY] = DxiTxy * DBx - APDx * ai[TXY];
-            ha[TXZ] = DxiTxz * DBx - APDx * ai[TXZ];
-            ha[TYZ] = DxiTyz * DBx - APDx * ai[TYZ];
-            if (top == 2) {
-                ha[TXX] += DBx * Bx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
-                ha[TYY] += DBx * Bx * (lambda * ztx * DzVx1 + lam2mu * zty * DzVy1 + lambda * ztz * DzVz1);
-                ha[TZZ] += DBx * Bx * (lambda * ztx * DzVx1 + lambda * zty * DzVy1 + lam2mu * ztz * DzVz1);
-                ha[TXY] += DBx * Bx * (zty * DzVx1 + ztx * DzVy1) * miu;
-                ha[TXZ] += DBx * Bx * (ztz * DzVx1 + ztx * DzVz1) * miu;
-                ha[TYZ] += DBx * Bx * (ztz * DzVy1 + zty * DzVz1) * miu;
-            }
-            if (top >= 0) {
-#pragma unroll
-                for (int c = VX; c <= VZ; ++c) ha[c] = a.timg[(3 + c) * a.tstr + tcell];
-            }
-            CGFD_AUX_UPDATE(0, ha, ai)
-        }
-        if (iny) {
-            const long aidx = ((long)z * a.npml[1] + iy) * row + x;
-            R ai[NF], ha[NF];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) ai[c] = a.ain[1][c * a.astr[1] + aidx];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / By;
-            ha[VX] = DetVx * DBy - APDy * ai[VX];
-            ha[VY] = DetVy * DBy - APDy * ai[VY];
-            ha[VZ] = DetVz * DBy - APDy * ai[VZ];
-            ha[TXX] = DetTxx * DBy - APDy * ai[TXX];
-            ha[TYY] = DetTyy * DBy - APDy * ai[TYY];
-            ha[TZZ] = DetTzz * DBy - APDy * ai[TZZ];
-            ha[TXY] = DetTxy * DBy - APDy * ai[TXY];
-            ha[TXZ] = DetTxz * DBy - APDy * ai[TXZ];
-            ha[TYZ] = DetTyz * DBy - APDy * ai[TYZ];
-            if (top == 2) {
-                ha[TXX] += DBy * By * (lam2mu * ztx * DzVx2 + lambda * zty * DzVy2 + lambda * ztz * DzVz2);
-                ha[TYY] += DBy * By * (lambda * ztx * DzVx2 + lam2mu * zty * DzVy2 + lambda * ztz * DzVz2);
-                ha[TZZ] += DBy * By * (lambda * ztx * DzVx2 + lambda * zty * DzVy2 + lam2mu * ztz * DzVz2);
-                ha[TXY] += DBy * By * (zty * DzVx2 + ztx * DzVy2) * miu;
-                ha[TXZ] += DBy * By * (ztz * DzVx2 + ztx * DzVz2) * miu;
-                ha[TYZ] += DBy * By * (ztz * DzVy2 + zty * DzVz2) * miu;
-            }
-            if (top >= 0) {
-#pragma unroll
-                for (int c = VX; c <= VZ; ++c) ha[c] = a.timg[(6 + c) * a.tstr + tcell];
-            }
-            CGFD_AUX_UPDATE(1, ha, ai)
-        }
-        const bool inz = z < a.nabs[4] || z >= g.nk - a.nabs[5];
-        if (inz) {
-            const int iz = z < a.nabs[4] ? z : z - (g.nk - a.nabs[5]) + a.nabs[4];
-            const long aidx = ((long)iz * g.nj + y) * row + x;
-            R ai[NF], ha[NF];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) ai[c] = a.ain[2][c * a.astr[2] + aidx];
-#pragma unroll
-            for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bz;
-            ha[VX] = DztVx * DBz - APDz * ai[VX];
-            ha[VY] = DztVy * DBz - APDz * ai[VY];
-            ha[VZ] = DztVz * DBz - APDz * ai[VZ];
-            ha[TXX] = DztTxx * DBz - APDz * ai[TXX];
-            ha[TYY] = DztTyy * DBz - APDz * ai[TYY];
-            ha[TZZ] = DztTzz * DBz - APDz * ai[TZZ];
-            ha[TXY] = DztTxy * DBz - APDz * ai[TXY];
-            ha[TXZ] = DztTxz * DBz - APDz * ai[TXZ];
-            ha[TYZ] = DztTyz * DBz - APDz * ai[TYZ];
-            CGFD_AUX_UPDATE(2, ha, ai)
-        }
-#undef CGFD_AUX_UPDATE
-
-        // ---- CalTIMG override of the velocity rates on the top three rows -----
-        if (top >= 0) {
-#pragma unroll
-            for (int c = VX; c <= VZ; ++c) hv[c] = a.timg[c * a.tstr + tcell];
-        }
-
-        // ---- LoadForce then LoadMoment, in source order ------------------------
-        if (e >= 0) {
-            int4 en = a.sent[e];
-            while (en.w == col && en.x == z) {
-                const R *cf = a.scoef + (long)e * 8;
-                if (en.y == 0) {
-                    const R stf = a.fstf[(long)en.z * a.nstf_f + a.tindex];
-                    const R A = cf[0];
-                    hv[VX] += stf * cf[1] * A;
-                    hv[VY] += stf * cf[2] * A;
-                    hv[VZ] += stf * cf[3] * A;
-                } else {
-                    const R stf = a.mstf[(long)en.z * a.nstf_m + a.tindex];
-                    R A = (R)1;
-                    A = stf * A / cf[0];
-                    hv[TXX] -= cf[1] * A;
-                    hv[TYY] -= cf[2] * A;
-                    hv[TZZ] -= cf[3] * A;
-                    hv[TXY] -= cf[4] * A;
-                    hv[TXZ] -= cf[5] * A;
-                    hv[TYZ] -= cf[6] * A;
-                }
-                en = a.sent[++e];
-            }
-        }
-
-        // ---- RK4 update (Iteration*, calculate.cu:6330-6852) -------------------
-#pragma unroll
-        for (int c = 0; c < NF; ++c) {
-            const long q = c * fs + p;
-            if (ST == 0) {
-                const R f0 = w[c][CI]; // the stage input is FW itself
-                a.wout[q] = f0 + a.alpha * hv[c];
-                a.tw[q] = f0 + a.beta * hv[c];
-            } else if (ST < 3) {
-                a.wout[q] = a.fw[q] + a.alpha * hv[c];
-                a.tw[q] = a.tw[q] + a.beta * hv[c];
-            } else {
-                const R wn = a.tw[q] + a.beta * hv[c];
-                a.wout[q] = wn;
-                if (c < 3 && top == 2 && a.pgv) {
-                    R *pv = a.pgv + c * a.cstr + cxy; // PGV maps share the [nj][nxp] slot size of cov
-                    *pv = rmax(rabs(wn), rabs(*pv));
-                }
-            }
-        }
-
-        // ---- slide the z window ---------------------------------------------------
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             w[c][0] = w[c][1];
             w[c][1] = w[c][2];
             w[c][2] = w[c][3];
             w[c][3] = w[c][4];
+        }
+    }
+}
+
+// Variant "lds": xy-plane staging.  A 256-lane workgroup owns a 64 x 4 tile
+// of (x,y) columns and sweeps a chunk of z planes.  Per plane every lane
+// writes its cell of the 9 stage-input components into an LDS tile that also
+// holds the one-sided halo (1 cell behind and 3 ahead for forward
+// differences, 3 behind and 1 ahead for backward), so all 18 in-plane
+// stencils of a cell are LDS reads at immediate offsets from one address.
+// The z taps stay in the per-lane 5-deep register window; the next plane's
+// global loads (window lead and halo) are issued before the current plane's
+// arithmetic.  Tiles are numbered so that consecutive tiles of one XCD are
+// spatial neighbours (shared halo lines stay in that XCD's L2).
+constexpr int LDS_TY = 4;
+#ifndef CGFD_LDS_WAVES
+#define CGFD_LDS_WAVES 2
+#endif
+#define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
+
+template <typename R, int SX, int SZ, int ST>
+__global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tindex)
+{
+    // The argument block lives in device memory and is read through the
+    // constant address space, so fields are fetched with scalar loads where
+    // they are used instead of being preloaded into (and spilled from) SGPRs.
+    typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
+    const AP a0 = (AP)pa;
+    constexpr int TY = LDS_TY, HX = 64 + 4, HY = TY + 4;
+    constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
+    constexpr int ZOFF = SZ > 0 ? -1 : -3;
+    constexpr int CI = SZ > 0 ? 1 : 3;
+    __shared__ R lds[NF][HY][HX];
+
+    const AP a = a0;
+    const Geom g = geom_of(a);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+
+    // XCD-aware tile numbering: workgroups are dealt round-robin to the 8
+    // XCDs; give each XCD a contiguous run of tiles
+    const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
+    const int nb = gridDim.x, id = blockIdx.x;
+    const int q = nb / 8, r = nb % 8, xcd = id % 8;
+    const int L = xcd * q + min(xcd, r) + id / 8;
+    const int bx = L % ntx, by = (L / ntx) % nty, bz = L / (ntx * nty);
+    const int x0 = bx * 64, y0 = by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < g.ni && y < g.nj;
+    const int z0 = bz * a->zlen;
+    const int z1 = min(g.nk, z0 + a->zlen);
+    if (z0 >= z1) return; // uniform over the workgroup
+
+    const long fs0 = g.fs;
+    const double h = a->h, rh = a->rh;
+    const unsigned esz = sizeof(R);
+    const unsigned plb = (unsigned)(g.plane * esz); // bytes per plane
+
+    // halo slots of this lane: one of the 4 halo rows (all 64 columns), and
+    // for lanes 0..15 one of the 4 halo columns (the 4 tile rows)
+    const int hk = tid >> 6;
+    const int hrow = SX > 0 ? (hk == 0 ? 0 : TY + hk) : (hk < 3 ? hk : TY + 3);
+    const int hy = y0 - O + hrow, hxx = x0 + tx;
+    const bool hv1 = hy >= -3 && hy < g.nj + 3 && hxx < g.ni + 3;
+    const int hc2k = tid >> 2, hr2 = tid & 3;
+    const int hcol = SX > 0 ? (hc2k == 0 ? 0 : 64 + hc2k) : (hc2k < 3 ? hc2k : 67);
+    const int hx2 = x0 - O + hcol, hy2 = y0 + hr2;
+    const bool hv2 = tid < 16 && hx2 >= -3 && hx2 < g.ni + 3 && hy2 < g.nj + 3;
+    // byte offsets inside plane 0 (may be negative: added to a plane base)
+    const int rel_c = (y * g.nxp + x) * (int)esz;
+    const int rel_1 = (hy * g.nxp + hxx) * (int)esz;
+    const int rel_2 = (hy2 * g.nxp + hx2) * (int)esz;
+    const unsigned base0 = (unsigned)(g.base * esz);
+    auto pbase = [&](int zz) { return base0 + (unsigned)zz * plb; };
+
+    const Column<R> k = column_of<R>(a, active ? x : 0, active ? y : 0);
+    int e = active ? first_source(a, k.col, z0) : -1;
+
+    R w[NF][5];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int c = 0; c < NF; ++c) w[c][j] = ldg(a->win + c * fs0, pbase(z0 + ZOFF + j) + rel_c);
+
+    R lead[NF], h1[NF], h2[NF];
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        lead[c] = ldg(a->win + c * fs0, pbase(z0 + ZOFF + 4) + rel_c);
+        h1[c] = hv1 ? ldg(a->win + c * fs0, pbase(z0) + rel_1) : (R)0;
+        h2[c] = hv2 ? ldg(a->win + c * fs0, pbase(z0) + rel_2) : (R)0;
+    }
+
+    for (int z = z0; z < z1; ++z) {
+        // The field stride is laundered through an empty asm once per plane,
+        // so the ~40 per-field base addresses derived from it are rebuilt on
+        // the scalar unit each plane instead of being held (and spilled)
+        // across the whole sweep.
+        long fs = g.fs;
+        asm volatile("" : "+s"(fs));
+        AP al = a0;
+        asm volatile("" : "+s"(al));
+        const unsigned pbz = pbase(z) + rel_c;
+        CellIn<R> cin;
+        if (active) load_cell<R, ST>(al, pbz, fs, cin);
+        __syncthreads(); // the previous plane's LDS reads are done
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+            lds[c][O + ty][O + tx] = w[c][CI];
+            lds[c][hrow][O + tx] = h1[c];
+            if (tid < 16) lds[c][O + hr2][hcol] = h2[c];
+            w[c][4] = lead[c];
+        }
+        __syncthreads();
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int c = 0; c < NF; ++c) {
+                lead[c] = ldg(al->win + c * fs, pbase(z + 1 + ZOFF + 4) + rel_c);
+                h1[c] = hv1 ? ldg(al->win + c * fs, pbase(z + 1) + rel_1) : (R)0;
+                h2[c] = hv2 ? ldg(al->win + c * fs, pbase(z + 1) + rel_2) : (R)0;
+            }
+        }
+        if (active) {
+            R Dx[NF], Dy[NF], Dz[NF];
+#pragma unroll
+            for (int c = 0; c < NF; ++c) {
+                const R ctr = w[c][CI];
+                const R *rx = &lds[c][O + ty][tx]; // taps x-O .. x-O+4
+                const R *ry = &lds[c][ty][O + tx]; // taps y-O .. y-O+4
+                if (SX > 0) {
+                    Dx[c] = dfd<R, 1>(rx[0], ctr, rx[2], rx[3], rx[4], h, rh);
+                    Dy[c] = dfd<R, 1>(ry[0], ctr, ry[2 * HX], ry[3 * HX], ry[4 * HX], h, rh);
+                } else {
+                    Dx[c] = dfd<R, -1>(rx[0], rx[1], rx[2], ctr, rx[4], h, rh);
+                    Dy[c] = dfd<R, -1>(ry[0], ry[HX], ry[2 * HX], ctr, ry[4 * HX], h, rh);
+                }
+                Dz[c] = dfd<R, SZ>(w[c][0], w[c][1], w[c][2], w[c][3], w[c][4], h, rh);
+            }
+            finish_cell<R, SX, SZ, ST>(al, k, cin, Dx, Dy, Dz, w, x, y, z, pbz, e, fs, tindex);
+        }
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+            w[c][0] = w[c][1];
+            w[c][1] = w[c][2];
+            w[c][2] = w[c][3];
+            w[c][3] = w[c][4];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant "split": each stage as two kernels, one per half of the system.
+//   PART 0 (velocity): derivatives of the 6 stresses -> momentum rates of
+//     Vx, Vy, Vz, their PML auxiliaries, the traction-image override, point
+//     forces, the RK update of V and the PGV maximum;
+//   PART 1 (stress): derivatives of the 3 velocities (with the free-surface
+//     closure of the top rows) -> Hooke rates of the 6 stresses, their PML
+//     auxiliaries (including the surface terms), moment sources and the RK
+//     update of T.
+// Within a stage the two halves only read the stage input, so they are
+// independent.  Each half keeps roughly half the live state of the fused
+// kernel, which doubles the resident waves; the price is that both read the
+// nine metrics.  Arithmetic per component is exactly that of finish_cell.
+// ---------------------------------------------------------------------------
+#ifndef CGFD_SPLIT_WAVES_V
+#define CGFD_SPLIT_WAVES_V 4
+#endif
+#ifndef CGFD_SPLIT_WAVES_T
+#define CGFD_SPLIT_WAVES_T 4
+#endif
+
+template <int PART> struct SplitFields;
+template <> struct SplitFields<0> { // stencil inputs: the six stresses
+    static constexpr int n = 6;
+    __device__ static constexpr int comp(int i) { return TXX + i; }
+};
+template <> struct SplitFields<1> { // stencil inputs: the three velocities
+    static constexpr int n = 3;
+    __device__ static constexpr int comp(int i) { return VX + i; }
+};
+
+template <typename R, int SX, int SZ, int ST, int PART>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (PART == 0 ? CGFD_SPLIT_WAVES_V : CGFD_SPLIT_WAVES_T) : 2, 8)))
+stage_split_kernel(const StageArgs<R> *pa, int tindex)
+{
+    typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
+    constexpr int NIN = SplitFields<PART>::n;
+    constexpr int TY = LDS_TY, HX = 64 + 4, HY = TY + 4;
+    constexpr int O = SX > 0 ? 1 : 3;
+    constexpr int ZOFF = SZ > 0 ? -1 : -3;
+    constexpr int CI = SZ > 0 ? 1 : 3;
+    __shared__ R lds[NIN][HY][HX];
+
+    const AP a0 = (AP)pa;
+    const Geom g = geom_of(a0);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
+    const int nb = gridDim.x, id = blockIdx.x;
+    const int q = nb / 8, r = nb % 8, xcd = id % 8;
+    const int L = xcd * q + min(xcd, r) + id / 8;
+    const int bx = L % ntx, by = (L / ntx) % nty, bz = L / (ntx * nty);
+    const int x0 = bx * 64, y0 = by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < g.ni && y < g.nj;
+    const int zlen = a0->zlen;
+    const int z0 = bz * zlen;
+    const int z1 = min(g.nk, z0 + zlen);
+    if (z0 >= z1) return;
+
+    const double h = a0->h, rh = a0->rh;
+    const unsigned esz = sizeof(R);
+    const unsigned plb = (unsigned)(g.plane * esz);
+    const int hk = tid >> 6;
+    const int hrow = SX > 0 ? (hk == 0 ? 0 : TY + hk) : (hk < 3 ? hk : TY + 3);
+    const int hy = y0 - O + hrow, hxx = x0 + tx;
+    const bool hv1 = hy >= -3 && hy < g.nj + 3 && hxx < g.ni + 3;
+    const int hc2k = tid >> 2, hr2 = tid & 3;
+    const int hcol = SX > 0 ? (hc2k == 0 ? 0 : 64 + hc2k) : (hc2k < 3 ? hc2k : 67);
+    const int hx2 = x0 - O + hcol, hy2 = y0 + hr2;
+    const bool hv2 = tid < 16 && hx2 >= -3 && hx2 < g.ni + 3 && hy2 < g.nj + 3;
+    const int rel_c = (y * g.nxp + x) * (int)esz;
+    const int rel_1 = (hy * g.nxp + hxx) * (int)esz;
+    const int rel_2 = (hy2 * g.nxp + hx2) * (int)esz;
+    const unsigned base0 = (unsigned)(g.base * esz);
+    auto pbase = [&](int zz) { return base0 + (unsigned)zz * plb; };
+
+    const Column<R> k = column_of<R>(a0, active ? x : 0, active ? y : 0);
+    int e = active ? first_source(a0, k.col, z0) : -1;
+
+    const long fs0 = g.fs;
+    R w[NIN][5];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < NIN; ++i)
+            w[i][j] = ldg(a0->win + SplitFields<PART>::comp(i) * fs0, pbase(z0 + ZOFF + j) + rel_c);
+    R lead[NIN], h1[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+        const R *f = a0->win + SplitFields<PART>::comp(i) * fs0;
+        lead[i] = ldg(f, pbase(z0 + ZOFF + 4) + rel_c);
+        h1[i] = hv1 ? ldg(f, pbase(z0) + rel_1) : (R)0;
+    }
+
+    for (int z = z0; z < z1; ++z) {
+        long fs = g.fs;
+        asm volatile("" : "+s"(fs));
+        AP a = a0;
+        asm volatile("" : "+s"(a));
+        const unsigned pb = pbase(z) + rel_c;
+        const int top = z - (g.nk - 3);
+
+        // per-cell inputs, issued before the stencil work
+        R m[9];
+        R med0 = 0, med1 = 0;
+        constexpr int NO = PART == 0 ? 3 : 6; // output components
+        constexpr int C0 = PART == 0 ? VX : TXX;
+        R fw[NO], tw[NO];
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) m[i] = ldg(a->met + i * fs, pb);
+            if (PART == 0) {
+                med0 = ldg(a->med + 2 * fs, pb); // 1/rho
+            } else {
+                med0 = ldg(a->med, pb);      // lam2mu
+                med1 = ldg(a->med + fs, pb); // miu
+            }
+#pragma unroll
+            for (int c = 0; c < NO; ++c) {
+                if (ST == 1 || ST == 2) fw[c] = ldg(a->fw + (C0 + c) * fs, pb);
+                if (ST >= 1) tw[c] = ldg((const R *)a->tw + (C0 + c) * fs, pb);
+            }
+        }
+        // the x-halo column values of this plane (16 lanes)
+        R h2[NIN];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) h2[i] = hv2 ? ldg(a->win + SplitFields<PART>::comp(i) * fs, pbase(z) + rel_2) : (R)0;
+
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) {
+            lds[i][O + ty][O + tx] = w[i][CI];
+            lds[i][hrow][O + tx] = h1[i];
+            if (tid < 16) lds[i][O + hr2][hcol] = h2[i];
+            w[i][4] = lead[i];
+        }
+        __syncthreads();
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int i = 0; i < NIN; ++i) {
+                const R *f = a->win + SplitFields<PART>::comp(i) * fs;
+                lead[i] = ldg(f, pbase(z + 1 + ZOFF + 4) + rel_c);
+                h1[i] = hv1 ? ldg(f, pbase(z + 1) + rel_1) : (R)0;
+            }
+        }
+
+        if (active) {
+            R Dx[NIN], Dy[NIN], Dz[NIN];
+#pragma unroll
+            for (int i = 0; i < NIN; ++i) {
+                const R ctr = w[i][CI];
+                const R *rx = &lds[i][O + ty][tx];
+                const R *ry = &lds[i][ty][O + tx];
+                if (SX > 0) {
+                    Dx[i] = dfd<R, 1>(rx[0], ctr, rx[2], rx[3], rx[4], h, rh);
+                    Dy[i] = dfd<R, 1>(ry[0], ctr, ry[2 * HX], ry[3 * HX], ry[4 * HX], h, rh);
+                } else {
+                    Dx[i] = dfd<R, -1>(rx[0], rx[1], rx[2], ctr, rx[4], h, rh);
+                    Dy[i] = dfd<R, -1>(ry[0], ry[HX], ry[2 * HX], ctr, ry[4 * HX], h, rh);
+                }
+                Dz[i] = dfd<R, SZ>(w[i][0], w[i][1], w[i][2], w[i][3], w[i][4], h, rh);
+            }
+            const R xix = m[XIX], xiy = m[XIY], xiz = m[XIZ];
+            const R etx = m[ETX], ety = m[ETY], etz = m[ETZ];
+            const R ztx = m[ZTX], zty = m[ZTY], ztz = m[ZTZ];
+            const R *pmz = a->pml + 3 * g.ni + 3 * g.nj;
+            const R APDz = pmz[z], Bz = pmz[g.nk + z], DBz = pmz[2 * g.nk + z];
+            const R Bx = k.Bx, By = k.By;
+            const bool unitB = Bx == (R)1 && By == (R)1 && Bz == (R)1;
+            const unsigned tcell = (unsigned)(top > 0 ? top : 0) * (unsigned)(g.nj * g.nxp) + k.cxy;
+            const bool inz = z < a->nabs[4] || z >= g.nk - a->nabs[5];
+            R hv[NO];
+            // directional terms: [0..NO) along xi, [NO..2NO) eta, [2NO..3NO) zeta
+            R Dxi[NO], Det[NO], Dzt[NO];
+            R DzVx1 = 0, DzVx2 = 0, DzVy1 = 0, DzVy2 = 0, DzVz1 = 0, DzVz2 = 0;
+            R lam2mu = 0, lambda = 0, miu = 0;
+
+            if constexpr (PART == 0) {
+                // CalWaveCL momentum terms (calculate.cu:4878-4890); Dx[i] is
+                // the xi-derivative of component TXX+i
+                const R rho = med0;
+                constexpr int XX = 0, YY = 1, ZZ = 2, XY = 3, XZ = 4, YZ = 5;
+                Dxi[0] = (xix * Dx[XX] + xiy * Dx[XY] + xiz * Dx[XZ]) * rho;
+                Det[0] = (etx * Dy[XX] + ety * Dy[XY] + etz * Dy[XZ]) * rho;
+                Dzt[0] = (ztx * Dz[XX] + zty * Dz[XY] + ztz * Dz[XZ]) * rho;
+                Dxi[1] = (xix * Dx[XY] + xiy * Dx[YY] + xiz * Dx[YZ]) * rho;
+                Det[1] = (etx * Dy[XY] + ety * Dy[YY] + etz * Dy[YZ]) * rho;
+                Dzt[1] = (ztx * Dz[XY] + zty * Dz[YY] + ztz * Dz[YZ]) * rho;
+                Dxi[2] = (xix * Dx[XZ] + xiy * Dx[YZ] + xiz * Dx[ZZ]) * rho;
+                Det[2] = (etx * Dy[XZ] + ety * Dy[YZ] + etz * Dy[ZZ]) * rho;
+                Dzt[2] = (ztx * Dz[XZ] + zty * Dz[YZ] + ztz * Dz[ZZ]) * rho;
+            } else {
+                // velocity zeta-derivatives of the top three rows (CondFreeVLOW,
+                // calculate.cu:4736-4769)
+                R cx[9], cy[9];
+                if (top == 2) {
+#pragma unroll
+                    for (int mm = 0; mm < 9; ++mm) {
+                        cx[mm] = a->cov[mm * a->cstr + k.cxy];
+                        cy[mm] = a->cov[(9 + mm) * a->cstr + k.cxy];
+                    }
+                    Dz[0] = cx[0] * Dx[0] + cx[1] * Dx[1] + cx[2] * Dx[2] + cy[0] * Dy[0] + cy[1] * Dy[1] + cy[2] * Dy[2];
+                    Dz[1] = cx[3] * Dx[0] + cx[4] * Dx[1] + cx[5] * Dx[2] + cy[3] * Dy[0] + cy[4] * Dy[1] + cy[5] * Dy[2];
+                    Dz[2] = cx[6] * Dx[0] + cx[7] * Dx[1] + cx[8] * Dx[2] + cy[6] * Dy[0] + cy[7] * Dy[1] + cy[8] * Dy[2];
+                    if (k.inx || k.iny) {
+                        DzVx1 = cx[0] * Dx[0] + cx[1] * Dx[1] + cx[2] * Dx[2];
+                        DzVx2 = cy[0] * Dy[0] + cy[1] * Dy[1] + cy[2] * Dy[2];
+                        DzVy1 = cx[3] * Dx[0] + cx[4] * Dx[1] + cx[5] * Dx[2];
+                        DzVy2 = cy[3] * Dy[0] + cy[4] * Dy[1] + cy[5] * Dy[2];
+                        DzVz1 = cx[6] * Dx[0] + cx[7] * Dx[1] + cx[8] * Dx[2];
+                        DzVz2 = cy[6] * Dy[0] + cy[7] * Dy[1] + cy[8] * Dy[2];
+                    }
+                } else if (top == 1) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        Dz[c] = SZ > 0 ? (w[c][2] - w[c][1]) / a->hr : (w[c][3] - w[c][2]) / a->hr;
+                } else if (top == 0) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double s = SZ > 0 ? M24A0 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A2 * (double)w[c][3]
+                                          : M24A2 * (double)w[c][1] + M24A1 * (double)w[c][2] + M24A0 * (double)w[c][3];
+                        Dz[c] = (R)(SZ > 0 ? s / h : s / (-h));
+                    }
+                }
+                // CalWaveCL Hooke terms (calculate.cu:4892-4920)
+                lam2mu = med0;
+                miu = med1;
+                lambda = lam2mu - (R)2 * miu;
+                Dxi[0] = lam2mu * xix * Dx[0] + lambda * xiy * Dx[1] + lambda * xiz * Dx[2];
+                Det[0] = lam2mu * etx * Dy[0] + lambda * ety * Dy[1] + lambda * etz * Dy[2];
+                Dzt[0] = lam2mu * ztx * Dz[0] + lambda * zty * Dz[1] + lambda * ztz * Dz[2];
+                Dxi[1] = lambda * xix * Dx[0] + lam2mu * xiy * Dx[1] + lambda * xiz * Dx[2];
+                Det[1] = lambda * etx * Dy[0] + lam2mu * ety * Dy[1] + lambda * etz * Dy[2];
+                Dzt[1] = lambda * ztx * Dz[0] + lam2mu * zty * Dz[1] + lambda * ztz * Dz[2];
+                Dxi[2] = lambda * xix * Dx[0] + lambda * xiy * Dx[1] + lam2mu * xiz * Dx[2];
+                Det[2] = lambda * etx * Dy[0] + lambda * ety * Dy[1] + lam2mu * etz * Dy[2];
+                Dzt[2] = lambda * ztx * Dz[0] + lambda * zty * Dz[1] + lam2mu * ztz * Dz[2];
+                Dxi[3] = (xiy * Dx[0] + xix * Dx[1]) * miu;
+                Det[3] = (ety * Dy[0] + etx * Dy[1]) * miu;
+                Dzt[3] = (zty * Dz[0] + ztx * Dz[1]) * miu;
+                Dxi[4] = (xiz * Dx[0] + xix * Dx[2]) * miu;
+                Det[4] = (etz * Dy[0] + etx * Dy[2]) * miu;
+                Dzt[4] = (ztz * Dz[0] + ztx * Dz[2]) * miu;
+                Dxi[5] = (xiz * Dx[1] + xiy * Dx[2]) * miu;
+                Det[5] = (etz * Dy[1] + ety * Dy[2]) * miu;
+                Dzt[5] = (ztz * Dz[1] + zty * Dz[2]) * miu;
+            }
+#pragma unroll
+            for (int c = 0; c < NO; ++c)
+                hv[c] = unitB ? Dxi[c] + Det[c] + Dzt[c] : Dxi[c] / Bx + Det[c] / By + Dzt[c] / Bz;
+
+            // ---- ADE CFS-PML (calculate.cu:4993-5107) + auxiliary RK update ----
+            auto aux_update = [&](int AX, long aidx, const R *ai, const R *ha) {
+                const long as = a->astr[AX];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) {
+                    const long o = (C0 + c) * as + aidx;
+                    if (ST == 0) {
+                        a->aout[AX][o] = ai[c] + a->alpha * ha[c];
+                        a->at[AX][o] = ai[c] + a->beta * ha[c];
+                    } else if (ST < 3) {
+                        a->aout[AX][o] = a->af[AX][o] + a->alpha * ha[c];
+                        a->at[AX][o] = a->at[AX][o] + a->beta * ha[c];
+                    } else {
+                        a->aout[AX][o] = a->at[AX][o] + a->beta * ha[c];
+                    }
+                }
+            };
+            if (k.inx) {
+                const long aidx = ((long)z * g.nj + y) * a->npml[0] + k.ix;
+                R ai[NO], ha[NO];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ai[c] = a->ain[0][(C0 + c) * a->astr[0] + aidx];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) hv[c] -= ai[c] / Bx;
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ha[c] = Dxi[c] * k.DBx - k.APDx * ai[c];
+                if constexpr (PART == 1) {
+                    if (top == 2) {
+                        const R DBx = k.DBx;
+                        ha[0] += DBx * Bx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
+                        ha[1] += DBx * Bx * (lambda * ztx * DzVx1 + lam2mu * zty * DzVy1 + lambda * ztz * DzVz1);
+                        ha[2] += DBx * Bx * (lambda * ztx * DzVx1 + lambda * zty * DzVy1 + lam2mu * ztz * DzVz1);
+                        ha[3] += DBx * Bx * (zty * DzVx1 + ztx * DzVy1) * miu;
+                        ha[4] += DBx * Bx * (ztz * DzVx1 + ztx * DzVz1) * miu;
+                        ha[5] += DBx * Bx * (ztz * DzVy1 + zty * DzVz1) * miu;
+                    }
+                } else {
+                    if (top >= 0) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) ha[c] = a->timg[(3 + c) * a->tstr + tcell];
+                    }
+                }
+                aux_update(0, aidx, ai, ha);
+            }
+            if (k.iny) {
+                const long aidx = ((long)z * a->npml[1] + k.iy) * g.nxp + x;
+                R ai[NO], ha[NO];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ai[c] = a->ain[1][(C0 + c) * a->astr[1] + aidx];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) hv[c] -= ai[c] / By;
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ha[c] = Det[c] * k.DBy - k.APDy * ai[c];
+                if constexpr (PART == 1) {
+                    if (top == 2) {
+                        const R DBy = k.DBy;
+                        ha[0] += DBy * By * (lam2mu * ztx * DzVx2 + lambda * zty * DzVy2 + lambda * ztz * DzVz2);
+                        ha[1] += DBy * By * (lambda * ztx * DzVx2 + lam2mu * zty * DzVy2 + lambda * ztz * DzVz2);
+                        ha[2] += DBy * By * (lambda * ztx * DzVx2 + lambda * zty * DzVy2 + lam2mu * ztz * DzVz2);
+                        ha[3] += DBy * By * (zty * DzVx2 + ztx * DzVy2) * miu;
+                        ha[4] += DBy * By * (ztz * DzVx2 + ztx * DzVz2) * miu;
+                        ha[5] += DBy * By * (ztz * DzVy2 + zty * DzVz2) * miu;
+                    }
+                } else {
+                    if (top >= 0) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) ha[c] = a->timg[(6 + c) * a->tstr + tcell];
+                    }
+                }
+                aux_update(1, aidx, ai, ha);
+            }
+            if (inz) {
+                const int iz = z < a->nabs[4] ? z : z - (g.nk - a->nabs[5]) + a->nabs[4];
+                const long aidx = ((long)iz * g.nj + y) * g.nxp + x;
+                R ai[NO], ha[NO];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ai[c] = a->ain[2][(C0 + c) * a->astr[2] + aidx];
+#pragma unroll
+                for (int c = 0; c < NO; ++c) hv[c] -= ai[c] / Bz;
+#pragma unroll
+                for (int c = 0; c < NO; ++c) ha[c] = Dzt[c] * DBz - APDz * ai[c];
+                aux_update(2, aidx, ai, ha);
+            }
+
+            if constexpr (PART == 0) {
+                if (top >= 0) { // CalTIMG override
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) hv[c] = a->timg[c * a->tstr + tcell];
+                }
+            }
+
+            // ---- point sources of this half, in source order ----------------------
+            if (e >= 0) {
+                int4 en = a->sent[e];
+                while (en.w == k.col && en.x == z) {
+                    const R *cf = a->scoef + (long)e * 8;
+                    if (PART == 0 && en.y == 0) {
+                        const R stf = a->fstf[(long)en.z * a->nstf_f + tindex];
+                        const R A = cf[0];
+                        hv[0] += stf * cf[1] * A;
+                        hv[1] += stf * cf[2] * A;
+                        hv[2] += stf * cf[3] * A;
+                    } else if (PART == 1 && en.y == 1) {
+                        const R stf = a->mstf[(long)en.z * a->nstf_m + tindex];
+                        R A = (R)1;
+                        A = stf * A / cf[0];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) hv[c] -= cf[1 + c] * A;
+                    }
+                    en = a->sent[++e];
+                }
+            }
+
+            // ---- RK4 update ----------------------------------------------------------
+#pragma unroll
+            for (int c = 0; c < NO; ++c) {
+                R *wo = a->wout + (C0 + c) * fs;
+                R *twp = a->tw + (C0 + c) * fs;
+                if (ST == 0) {
+                    // the stage input is FW itself; for the stencil components it
+                    // is already in the window, otherwise load it
+                    R f0;
+                    if (PART == 0) f0 = ldg(a->win + (VX + c) * fs, pb);
+                    else f0 = ldg(a->win + (TXX + c) * fs, pb);
+                    stg(wo, pb, f0 + a->alpha * hv[c]);
+                    stg(twp, pb, f0 + a->beta * hv[c]);
+                } else if (ST < 3) {
+                    stg(wo, pb, fw[c] + a->alpha * hv[c]);
+                    stg(twp, pb, tw[c] + a->beta * hv[c]);
+                } else {
+                    const R wn = tw[c] + a->beta * hv[c];
+                    stg(wo, pb, wn);
+                    if (PART == 0 && top == 2 && a->pgv) {
+                        R *pv = a->pgv + c * a->cstr + k.cxy;
+                        *pv = rmax(rabs(wn), rabs(*pv));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) {
+            w[i][0] = w[i][1];
+            w[i][1] = w[i][2];
+            w[i][2] = w[i][3];
+            w[i][3] = w[i][4];
         }
     }
 }
@@ -398,7 +1041,7 @@ __global__ void __launch_bounds__(256) timg_kernel(TimgArgs<R> a)
     for (int r = 0; r < 3; ++r) {
         const int z = g.nk - 3 + r;
         const long G = gidx(g, x, y, z);
-        const R rhojac = (R)(1.0 / (double)a.med[2 * fs + G] / (double)jac[G]);
+        const R rhojac = (R)(1.0 / (double)a.med[3 * fs + G] / (double)jac[G]);
         const long tcell = ((long)r * g.nj + y) * row + x;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -486,8 +1129,8 @@ __global__ void velpd_kernel(Geom g, const R *met, const R *med, const R *pml, R
     const R *pmx = pml, *pmy = pml + 3 * g.ni, *pmz = pmy + 3 * g.nj;
     const R Bzx = pmz[g.nk + z] / pmx[g.ni + x];
     const R Bzy = pmz[g.nk + z] / pmy[g.nj + y];
-    const R miu = med[2 * fs + G] * med[fs + G] * med[fs + G];
-    const R lam2mu = med[2 * fs + G] * med[G] * med[G];
+    const R miu = med[fs + G];    // rho*beta*beta
+    const R lam2mu = med[G];      // rho*alpha*alpha
     const R lambda = lam2mu - (R)2 * miu;
     const R e11 = met[XIX * fs + G], e12 = met[XIY * fs + G], e13 = met[XIZ * fs + G];
     const R e21 = met[ETX * fs + G], e22 = met[ETY * fs + G], e23 = met[ETZ * fs + G];
@@ -566,6 +1209,29 @@ template <typename R> __global__ void from_padded_kernel(const R *src, R *dst, G
 }
 
 // ---------------------------------------------------------------------------
+// media: alpha, beta, rho -> lam2mu, miu, 1/rho (slots 0-2), rho kept in slot 3
+// ---------------------------------------------------------------------------
+template <typename R> __global__ void media_prep_kernel(Geom g, R *med)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= g.ni) return;
+    const long G = gidx(g, x, y, z);
+    const R alpha = med[G], beta = med[g.fs + G], rho = med[3 * g.fs + G];
+    // calculate.cu:4855-4858: miu = rho*beta*beta, lam2mu = rho*alpha*alpha,
+    // and 1.0/rho (a double quotient of floats rounded to float equals the
+    // float quotient)
+    med[G] = rho * alpha * alpha;
+    med[g.fs + G] = rho * beta * beta;
+    med[2 * g.fs + G] = (R)1 / rho;
+}
+
+template <typename R> void launch_media_prep(const Geom &g, R *med, hipStream_t s)
+{
+    media_prep_kernel<R><<<dim3((g.ni + 255) / 256, g.nj, g.nk), dim3(256), 0, s>>>(g, med);
+}
+
+// ---------------------------------------------------------------------------
 // source coefficients (the step-independent parts of LoadForce/LoadMoment)
 // ---------------------------------------------------------------------------
 template <typename R>
@@ -583,7 +1249,7 @@ __global__ void src_prep_kernel(Geom g, const R *met, const R *med, const int4 *
         // calculate.cu:6173-6180
         R A = hr * hr * hr * jac;
         const R d = (R)1.0;
-        A = (R)1 / (A * med[2 * g.fs + G]) * d;
+        A = (R)1 / (A * med[3 * g.fs + G]) * d;
         if (z == g.nk - 1) A = A * (R)2;
         cf[0] = A;
         for (int k = 0; k < 3; ++k) cf[1 + k] = fvec[3 * en.z + k];
@@ -607,26 +1273,43 @@ __global__ void record_kernel(const R *fw, long fs, const long *rlin, int nrecv,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <typename R, int SX, int SZ, int ST>
+static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex, int rows,
+                             int zchunks, hipStream_t s)
+{
+    if (variant == 0) {
+        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + LDS_TY - 1) / LDS_TY;
+        stage_lds_kernel<R, SX, SZ, ST><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
+    } else if (variant == 2) {
+        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + LDS_TY - 1) / LDS_TY;
+        stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
+        stage_split_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
+    } else {
+        dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, zchunks);
+        stage_direct_kernel<R, SX, SZ, ST><<<grid, dim3(64, rows), 0, s>>>(pa, tindex);
+    }
+}
+
 template <typename R, int SX, int SZ>
-static void launch_stage_sxz(int st, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s)
+static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex, int rows,
+                             int zchunks, hipStream_t s)
 {
     switch (st) {
-    case 0: stage_kernel<R, SX, SZ, 0><<<grid, block, 0, s>>>(a); break;
-    case 1: stage_kernel<R, SX, SZ, 1><<<grid, block, 0, s>>>(a); break;
-    case 2: stage_kernel<R, SX, SZ, 2><<<grid, block, 0, s>>>(a); break;
-    default: stage_kernel<R, SX, SZ, 3><<<grid, block, 0, s>>>(a); break;
+    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, tindex, rows, zchunks, s); break;
+    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, tindex, rows, zchunks, s); break;
+    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, tindex, rows, zchunks, s); break;
+    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, tindex, rows, zchunks, s); break;
     }
 }
 
 template <typename R>
-void launch_stage(int sx, int sz, int st, const StageArgs<R> &a, int rows, int zchunks, hipStream_t s)
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex,
+                  int rows, int zchunks, hipStream_t s)
 {
-    dim3 block(64, rows);
-    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, zchunks);
-    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(st, a, grid, block, s);
-    else if (sx > 0) launch_stage_sxz<R, 1, -1>(st, a, grid, block, s);
-    else if (sz > 0) launch_stage_sxz<R, -1, 1>(st, a, grid, block, s);
-    else launch_stage_sxz<R, -1, -1>(st, a, grid, block, s);
+    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, tindex, rows, zchunks, s);
+    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, tindex, rows, zchunks, s);
+    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, tindex, rows, zchunks, s);
+    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, tindex, rows, zchunks, s);
 }
 
 template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s)
@@ -678,14 +1361,16 @@ void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, i
 }
 
 #define CGFD_INSTANTIATE(R)                                                                                   \
-    template void launch_stage<R>(int, int, int, const StageArgs<R> &, int, int, hipStream_t);              \
+    template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int,  \
+                                   hipStream_t);              \
     template void launch_timg<R>(int, int, const TimgArgs<R> &, int, hipStream_t);                          \
     template void launch_velpd<R>(const Geom &, const R *, const R *, const R *, R *, long, hipStream_t);     \
     template void launch_to_padded<R>(const R *, R *, const Geom &, hipStream_t);                            \
     template void launch_from_padded<R>(const R *, R *, const Geom &, hipStream_t);                          \
     template void launch_src_prep<R>(const Geom &, const R *, const R *, const int4 *, const R *, const R *,  \
                                      R *, int, R, hipStream_t);                                               \
-    template void launch_record<R>(const R *, long, const long *, int, R *, int, int, hipStream_t);
+    template void launch_record<R>(const R *, long, const long *, int, R *, int, int, hipStream_t);       \
+    template void launch_media_prep<R>(const Geom &, R *, hipStream_t);
 
 CGFD_INSTANTIATE(float)
 CGFD_INSTANTIATE(double)

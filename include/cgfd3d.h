@@ -130,6 +130,10 @@ int cgfd_profile_read(cgfd_ctx *ctx, double *ms_stage, long *n_stage, double *ms
 /* Launch shape of the fused stage kernel: z-chunks per column sweep and
  * rows per workgroup (0 keeps the current value). */
 int cgfd_set_tuning(cgfd_ctx *ctx, int zchunks, int rows);
+/* Stage kernel variant: 0 = xy planes staged in LDS (default), 1 = in-plane
+ * stencil taps loaded directly through the vector caches.  Both produce
+ * identical results. */
+int cgfd_set_variant(cgfd_ctx *ctx, int variant);
 /* Algorithmic HBM bytes of one RK stage of this configuration (the
  * 192 B/cell + 144 B/PML-cell-axis model, FP32; doubled for FP64). */
 double cgfd_stage_bytes(const cgfd_ctx *ctx);

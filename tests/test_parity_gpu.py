@@ -17,9 +17,13 @@ import oracle  # noqa: E402
 import scenario  # noqa: E402
 
 
-def _pair(sc, double=False):
+VARIANTS = [0, 1, 2]  # LDS-staged planes, direct in-plane loads, split velocity/stress halves
+
+
+def _pair(sc, double=False, variant=0):
     prec = cgfd3d.FP64 if double else cgfd3d.FP32
     g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], precision=prec)
+    g.set_variant(variant)
     o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], double=double)
     if double:
         sc = dict(sc)
@@ -53,12 +57,13 @@ def test_closure_matrices_match():
     assert_same(gy[s], oy[s], "CoVy")
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("phase", [0, 1, 2, 3])
-def test_random_state_each_stage(phase):
+def test_random_state_each_stage(phase, variant):
     """Random wavefield and auxiliary state, then one step stage by stage in
     each of the four F/B phases: every intermediate set matches."""
     sc = scenario.config_random(70, 37, 21, 8, [5, 4, 4, 5, 5, 0], seed=100 + phase)
-    g, o = _pair(sc)
+    g, o = _pair(sc, variant=variant)
     if phase:
         g.run(phase)
         o.run(phase)
@@ -83,11 +88,12 @@ def test_random_state_each_stage(phase):
         _compare_state(g, o, f"phase {phase} stage {k}")
 
 
-def test_random_scenario_run():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_random_scenario_run(variant):
     """Sources (incl. a doubled surface force and coincident sources),
     receivers at the surface and at depth, PML on five faces, 60 steps."""
     sc = scenario.config_random(50, 44, 30, 60, [6, 5, 4, 6, 6, 0], seed=7)
-    g, o = _pair(sc)
+    g, o = _pair(sc, variant=variant)
     g.run(60)
     o.run(60)
     assert_same(g.get_seismo(), o.get_seismo(), "seismograms")
@@ -95,22 +101,24 @@ def test_random_scenario_run():
     _compare_state(g, o, "after 60 steps")
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("nabs", [[0, 0, 0, 0, 0, 0], [7, 0, 0, 3, 5, 0], [3, 3, 3, 3, 3, 2]])
-def test_pml_layouts(nabs):
+def test_pml_layouts(nabs, variant):
     sc = scenario.config_random(33, 29, 18, 24, nabs, seed=sum(nabs) + 3)
-    g, o = _pair(sc)
+    g, o = _pair(sc, variant=variant)
     g.run(24)
     o.run(24)
     assert_same(g.get_seismo(), o.get_seismo(), f"seismograms nabs={nabs}")
     _compare_state(g, o, f"nabs={nabs}")
 
 
-def test_minimum_depth_grid():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_minimum_depth_grid(variant):
     """nk = 4: every row is one of the special top rows or the first below."""
     sc = scenario.config_random(17, 9, 4, 12, [2, 2, 1, 1, 0, 0], seed=5, nmom=1, nfrc=1, duplicate=False,
                                 topo_amp=20.0)
     assert np.isfinite(sc["metrics"]["jac"]).all() and (sc["metrics"]["jac"][3:-3, 3:-3, 3:-3] > 0).all()
-    g, o = _pair(sc)
+    g, o = _pair(sc, variant=variant)
     g.run(12)
     o.run(12)
     _compare_state(g, o, "nk=4")
@@ -119,8 +127,10 @@ def test_minimum_depth_grid():
 def test_tuning_does_not_change_results():
     sc = scenario.config_random(66, 30, 40, 16, [5, 5, 4, 4, 6, 0], seed=9)
     ref = None
-    for zc, rows in [(1, 4), (3, 2), (7, 1), (40, 4)]:
+    for variant, zc, rows in [(1, 1, 4), (1, 3, 2), (1, 7, 1), (1, 40, 4), (0, 1, 4), (0, 5, 4), (0, 40, 4),
+                              (2, 1, 4), (2, 6, 4), (2, 40, 4)]:
         g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+        g.set_variant(variant)
         g.set_tuning(zc, rows)
         g.setup(sc)
         g.run(16)
@@ -128,15 +138,16 @@ def test_tuning_does_not_change_results():
         if ref is None:
             ref = out
         else:
-            assert_same(out, ref, f"zchunks={zc} rows={rows}")
+            assert_same(out, ref, f"variant={variant} zchunks={zc} rows={rows}")
         g.close()
 
 
-def test_flat_config_full_run():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_flat_config_full_run(variant):
     """The small flat homogeneous configuration at full length (64^3, 600
     steps): receivers, snapshots every 100 steps and PGV."""
     sc = scenario.config_flat(nt=600)
-    g, o = _pair(sc)
+    g, o = _pair(sc, variant=variant)
     for step in range(100, 601, 100):
         g.run(100)
         o.run(100)
@@ -150,9 +161,10 @@ def test_flat_config_full_run():
     assert_same(g.get_pgv()[:, 3:-3, 3:-3], o.get_pgv()[:, 3:-3, 3:-3], "PGV")
 
 
-def test_fp64_build_matches_fp64_oracle():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_fp64_build_matches_fp64_oracle(variant):
     sc = scenario.config_random(40, 30, 24, 30, [5, 4, 4, 5, 5, 0], seed=21)
-    g, o = _pair(sc, double=True)
+    g, o = _pair(sc, double=True, variant=variant)
     g.run(30)
     o.run(30)
     assert_same(g.get_seismo(), o.get_seismo(), "FP64 seismograms")

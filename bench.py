@@ -77,6 +77,50 @@ def cpu_baseline(sc, steps):
                       f"problem on the C/OpenMP oracle, {threads} threads, {t:.1f} s"}
 
 
+def copy_calibration(mib, reps=3):
+    """A plain device-to-device copy of `mib` MiB, `reps` times: under a
+    counter profiler its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
+    counters against a known byte count (mib MiB read and written).  The copy
+    goes through the HIP runtime the engine library already loaded (a second
+    runtime in the same process, such as torch's bundled one, is avoided)."""
+    import ctypes
+
+    with open("/proc/self/maps") as f:
+        paths = {ln.split()[-1] for ln in f if "libamdhip64.so" in ln}
+    if not paths:
+        raise SystemExit("copy calibration: the HIP runtime is not loaded")
+    hip = ctypes.CDLL(sorted(paths)[0])
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise SystemExit(f"copy calibration: {what} returned {rc}")
+
+    ok(hip.hipMalloc(ctypes.byref(a), n), "hipMalloc")
+    ok(hip.hipMalloc(ctypes.byref(b), n), "hipMalloc")
+    ok(hip.hipMemset(a, 1, n), "hipMemset")
+    for _ in range(reps):
+        ok(hip.hipMemcpy(b, a, n, 3), "hipMemcpy")  # 3 = hipMemcpyDeviceToDevice
+    ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    ok(hip.hipFree(a), "hipFree")
+    ok(hip.hipFree(b), "hipFree")
+    log(f"copy calibration: {reps} x {mib} MiB device copy")
+
+
+def measured_traffic(workload, variant):
+    """HBM bytes per stage-kernel launch for this workload, from the committed
+    counter summary (profiles/stage_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "stage_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = table.get(f"{workload}/variant{variant}")
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +130,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=0, help="stage kernel: 0 = LDS-staged planes, 1 = direct loads")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="stage kernel: 0 = LDS-staged planes, 1 = direct loads, 2 = split velocity/stress")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed steps, run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -130,6 +177,8 @@ def main():
     finite = bool(abs(seis).max() < float("inf"))
     sim.close()
     del sim
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -155,7 +204,8 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(sc["name"], args.variant)},
         "cpu_baseline": cpu,
     }
     if cpu:

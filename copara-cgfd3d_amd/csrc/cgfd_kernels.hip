@@ -44,6 +44,45 @@ template <typename R> __device__ __forceinline__ void stg(R *base, unsigned off,
     *(R *)((char *)base + off) = v;
 }
 
+// Streams touched once per stage (media, metrics, the RK operands FW and tW,
+// every store) carry the nontemporal hint, so that the L2 keeps the stage
+// input planes, whose lines neighbouring tiles re-read as stencil halos.
+#ifndef CGFD_NT
+#define CGFD_NT 1
+#endif
+template <typename R> __device__ __forceinline__ R ldg_once(const R *base, unsigned off)
+{
+#if CGFD_NT
+    return __builtin_nontemporal_load((const R *)((const char *)base + off));
+#else
+    return ldg(base, off);
+#endif
+}
+template <typename R> __device__ __forceinline__ void stg_once(R *base, unsigned off, R v)
+{
+#if CGFD_NT
+    __builtin_nontemporal_store(v, (R *)((char *)base + off));
+#else
+    stg(base, off, v);
+#endif
+}
+template <typename R> __device__ __forceinline__ R ld_once(const R *p)
+{
+#if CGFD_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <typename R> __device__ __forceinline__ void st_once(R *p, R v)
+{
+#if CGFD_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // member-wise copy of the geometry out of a constant-address-space block
 template <typename AP> __device__ __forceinline__ Geom geom_of(AP a)
 {
@@ -112,15 +151,58 @@ template <typename R> struct CellIn {
 template <typename R, int ST, typename AP>
 __device__ __forceinline__ void load_cell(AP a, unsigned pb, long fs, CellIn<R> &in)
 {
-    in.lam2mu = ldg(a->med, pb);
-    in.miu = ldg(a->med + fs, pb);
-    in.rhoinv = ldg(a->med + 2 * fs, pb);
+    in.lam2mu = ldg_once(a->med, pb);
+    in.miu = ldg_once(a->med + fs, pb);
+    in.rhoinv = ldg_once(a->med + 2 * fs, pb);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) in.m[m] = ldg(a->met + m * fs, pb);
+    for (int m = 0; m < 9; ++m) in.m[m] = ldg_once(a->met + m * fs, pb);
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
-        if (ST == 1 || ST == 2) in.fw[c] = ldg(a->fw + c * fs, pb);
-        if (ST >= 1) in.tw[c] = ldg((const R *)a->tw + c * fs, pb);
+        if (ST == 1 || ST == 2) in.fw[c] = ldg_once(a->fw + c * fs, pb);
+        if (ST >= 1) in.tw[c] = ldg_once((const R *)a->tw + c * fs, pb);
+    }
+}
+
+// ADE auxiliary operands of one PML cell on one axis, all fetched before any
+// of the cell's auxiliary stores (the store targets may alias as far as the
+// compiler knows, so loads interleaved with stores would each wait for the
+// memory round trip).
+template <typename R> struct AuxIn {
+    R ai[NF]; // stage input A
+    R af[NF]; // step-start FA (stages 1-2)
+    R at[NF]; // accumulator tA (stages 1-3)
+};
+
+template <typename R, int ST, typename AP>
+__device__ __forceinline__ void load_aux(AP a, int ax, long aidx, AuxIn<R> &v)
+{
+    const long as = a->astr[ax];
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        v.ai[c] = ld_once(a->ain[ax] + c * as + aidx);
+        if (ST == 1 || ST == 2) v.af[c] = ld_once(a->af[ax] + c * as + aidx);
+        if (ST >= 1) v.at[c] = ld_once((const R *)a->at[ax] + c * as + aidx);
+    }
+}
+
+// RK update of the auxiliary fields (IterationA*, calculate.cu:6330-6852)
+template <typename R, int ST, typename AP>
+__device__ __forceinline__ void store_aux(AP a, int ax, long aidx, const AuxIn<R> &v, const R (&ha)[NF])
+{
+    const long as = a->astr[ax];
+    R *ao = a->aout[ax] + aidx;
+    R *at = a->at[ax] + aidx;
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        if (ST == 0) {
+            st_once(ao + c * as, v.ai[c] + a->alpha * ha[c]);
+            st_once(at + c * as, v.ai[c] + a->beta * ha[c]);
+        } else if (ST < 3) {
+            st_once(ao + c * as, v.af[c] + a->alpha * ha[c]);
+            st_once(at + c * as, v.at[c] + a->beta * ha[c]);
+        } else {
+            st_once(ao + c * as, v.at[c] + a->beta * ha[c]);
+        }
     }
 }
 
@@ -236,25 +318,6 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
     // ---- ADE CFS-PML (calculate.cu:4993-5107) + auxiliary RK update --------
     // The rates of one axis feed that axis' RK update straight away; the
     // traction image replaces the velocity rates of the top rows first.
-#define CGFD_AUX_UPDATE(AX, HA, AIN)                                                                      \
-    {                                                                                                     \
-        const long ap = aidx;                                                                             \
-        const long as = a->astr[AX];                                                                       \
-        _Pragma("unroll") for (int c = 0; c < NF; ++c)                                                    \
-        {                                                                                                 \
-            const R hh = HA[c];                                                                           \
-            if (ST == 0) {                                                                                \
-                a->aout[AX][c * as + ap] = AIN[c] + a->alpha * hh;                                          \
-                a->at[AX][c * as + ap] = AIN[c] + a->beta * hh;                                             \
-            } else if (ST < 3) {                                                                          \
-                a->aout[AX][c * as + ap] = a->af[AX][c * as + ap] + a->alpha * hh;                           \
-                a->at[AX][c * as + ap] = a->at[AX][c * as + ap] + a->beta * hh;                              \
-            } else {                                                                                      \
-                a->aout[AX][c * as + ap] = a->at[AX][c * as + ap] + a->beta * hh;                            \
-            }                                                                                             \
-        }                                                                                                 \
-    }
-
     R DzVx1 = 0, DzVx2 = 0, DzVy1 = 0, DzVy2 = 0, DzVz1 = 0, DzVz2 = 0;
     if (top == 2 && (k.inx || k.iny)) {
         // surface contributions routed into the x/y auxiliary stress rates
@@ -268,20 +331,20 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
     if (k.inx) {
         const R APDx = k.APDx, DBx = k.DBx;
         const long aidx = ((long)z * g.nj + y) * a->npml[0] + k.ix;
-        R ai[NF], ha[NF];
+        AuxIn<R> v;
+        load_aux<R, ST>(a, 0, aidx, v);
+        R ha[NF];
 #pragma unroll
-        for (int c = 0; c < NF; ++c) ai[c] = a->ain[0][c * a->astr[0] + aidx];
-#pragma unroll
-        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bx;
-        ha[VX] = DxiVx * DBx - APDx * ai[VX];
-        ha[VY] = DxiVy * DBx - APDx * ai[VY];
-        ha[VZ] = DxiVz * DBx - APDx * ai[VZ];
-        ha[TXX] = DxiTxx * DBx - APDx * ai[TXX];
-        ha[TYY] = DxiTyy * DBx - APDx * ai[TYY];
-        ha[TZZ] = DxiTzz * DBx - APDx * ai[TZZ];
-        ha[TXY] = DxiTxy * DBx - APDx * ai[TXY];
-        ha[TXZ] = DxiTxz * DBx - APDx * ai[TXZ];
-        ha[TYZ] = DxiTyz * DBx - APDx * ai[TYZ];
+        for (int c = 0; c < NF; ++c) hv[c] -= v.ai[c] / Bx;
+        ha[VX] = DxiVx * DBx - APDx * v.ai[VX];
+        ha[VY] = DxiVy * DBx - APDx * v.ai[VY];
+        ha[VZ] = DxiVz * DBx - APDx * v.ai[VZ];
+        ha[TXX] = DxiTxx * DBx - APDx * v.ai[TXX];
+        ha[TYY] = DxiTyy * DBx - APDx * v.ai[TYY];
+        ha[TZZ] = DxiTzz * DBx - APDx * v.ai[TZZ];
+        ha[TXY] = DxiTxy * DBx - APDx * v.ai[TXY];
+        ha[TXZ] = DxiTxz * DBx - APDx * v.ai[TXZ];
+        ha[TYZ] = DxiTyz * DBx - APDx * v.ai[TYZ];
         if (top == 2) {
             ha[TXX] += DBx * Bx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
             ha[TYY] += DBx * Bx * (lambda * ztx * DzVx1 + lam2mu * zty * DzVy1 + lambda * ztz * DzVz1);
@@ -294,25 +357,25 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
 #pragma unroll
             for (int c = VX; c <= VZ; ++c) ha[c] = a->timg[(3 + c) * a->tstr + tcell];
         }
-        CGFD_AUX_UPDATE(0, ha, ai)
+        store_aux<R, ST>(a, 0, aidx, v, ha);
     }
     if (k.iny) {
         const R APDy = k.APDy, DBy = k.DBy;
         const long aidx = ((long)z * a->npml[1] + k.iy) * g.nxp + x;
-        R ai[NF], ha[NF];
+        AuxIn<R> v;
+        load_aux<R, ST>(a, 1, aidx, v);
+        R ha[NF];
 #pragma unroll
-        for (int c = 0; c < NF; ++c) ai[c] = a->ain[1][c * a->astr[1] + aidx];
-#pragma unroll
-        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / By;
-        ha[VX] = DetVx * DBy - APDy * ai[VX];
-        ha[VY] = DetVy * DBy - APDy * ai[VY];
-        ha[VZ] = DetVz * DBy - APDy * ai[VZ];
-        ha[TXX] = DetTxx * DBy - APDy * ai[TXX];
-        ha[TYY] = DetTyy * DBy - APDy * ai[TYY];
-        ha[TZZ] = DetTzz * DBy - APDy * ai[TZZ];
-        ha[TXY] = DetTxy * DBy - APDy * ai[TXY];
-        ha[TXZ] = DetTxz * DBy - APDy * ai[TXZ];
-        ha[TYZ] = DetTyz * DBy - APDy * ai[TYZ];
+        for (int c = 0; c < NF; ++c) hv[c] -= v.ai[c] / By;
+        ha[VX] = DetVx * DBy - APDy * v.ai[VX];
+        ha[VY] = DetVy * DBy - APDy * v.ai[VY];
+        ha[VZ] = DetVz * DBy - APDy * v.ai[VZ];
+        ha[TXX] = DetTxx * DBy - APDy * v.ai[TXX];
+        ha[TYY] = DetTyy * DBy - APDy * v.ai[TYY];
+        ha[TZZ] = DetTzz * DBy - APDy * v.ai[TZZ];
+        ha[TXY] = DetTxy * DBy - APDy * v.ai[TXY];
+        ha[TXZ] = DetTxz * DBy - APDy * v.ai[TXZ];
+        ha[TYZ] = DetTyz * DBy - APDy * v.ai[TYZ];
         if (top == 2) {
             ha[TXX] += DBy * By * (lam2mu * ztx * DzVx2 + lambda * zty * DzVy2 + lambda * ztz * DzVz2);
             ha[TYY] += DBy * By * (lambda * ztx * DzVx2 + lam2mu * zty * DzVy2 + lambda * ztz * DzVz2);
@@ -325,28 +388,27 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
 #pragma unroll
             for (int c = VX; c <= VZ; ++c) ha[c] = a->timg[(6 + c) * a->tstr + tcell];
         }
-        CGFD_AUX_UPDATE(1, ha, ai)
+        store_aux<R, ST>(a, 1, aidx, v, ha);
     }
     if (z < a->nabs[4] || z >= g.nk - a->nabs[5]) {
         const int iz = z < a->nabs[4] ? z : z - (g.nk - a->nabs[5]) + a->nabs[4];
         const long aidx = ((long)iz * g.nj + y) * g.nxp + x;
-        R ai[NF], ha[NF];
+        AuxIn<R> v;
+        load_aux<R, ST>(a, 2, aidx, v);
+        R ha[NF];
 #pragma unroll
-        for (int c = 0; c < NF; ++c) ai[c] = a->ain[2][c * a->astr[2] + aidx];
-#pragma unroll
-        for (int c = 0; c < NF; ++c) hv[c] -= ai[c] / Bz;
-        ha[VX] = DztVx * DBz - APDz * ai[VX];
-        ha[VY] = DztVy * DBz - APDz * ai[VY];
-        ha[VZ] = DztVz * DBz - APDz * ai[VZ];
-        ha[TXX] = DztTxx * DBz - APDz * ai[TXX];
-        ha[TYY] = DztTyy * DBz - APDz * ai[TYY];
-        ha[TZZ] = DztTzz * DBz - APDz * ai[TZZ];
-        ha[TXY] = DztTxy * DBz - APDz * ai[TXY];
-        ha[TXZ] = DztTxz * DBz - APDz * ai[TXZ];
-        ha[TYZ] = DztTyz * DBz - APDz * ai[TYZ];
-        CGFD_AUX_UPDATE(2, ha, ai)
+        for (int c = 0; c < NF; ++c) hv[c] -= v.ai[c] / Bz;
+        ha[VX] = DztVx * DBz - APDz * v.ai[VX];
+        ha[VY] = DztVy * DBz - APDz * v.ai[VY];
+        ha[VZ] = DztVz * DBz - APDz * v.ai[VZ];
+        ha[TXX] = DztTxx * DBz - APDz * v.ai[TXX];
+        ha[TYY] = DztTyy * DBz - APDz * v.ai[TYY];
+        ha[TZZ] = DztTzz * DBz - APDz * v.ai[TZZ];
+        ha[TXY] = DztTxy * DBz - APDz * v.ai[TXY];
+        ha[TXZ] = DztTxz * DBz - APDz * v.ai[TXZ];
+        ha[TYZ] = DztTyz * DBz - APDz * v.ai[TYZ];
+        store_aux<R, ST>(a, 2, aidx, v, ha);
     }
-#undef CGFD_AUX_UPDATE
 
     // ---- CalTIMG override of the velocity rates on the top three rows -----
     if (top >= 0) {
@@ -385,14 +447,14 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
     for (int c = 0; c < NF; ++c) {
         if (ST == 0) {
             const R f0 = w[c][CI]; // the stage input is FW itself
-            stg(a->wout + c * fs, pb, f0 + a->alpha * hv[c]);
-            stg(a->tw + c * fs, pb, f0 + a->beta * hv[c]);
+            stg_once(a->wout + c * fs, pb, f0 + a->alpha * hv[c]);
+            stg_once(a->tw + c * fs, pb, f0 + a->beta * hv[c]);
         } else if (ST < 3) {
-            stg(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c]);
-            stg(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c]);
+            stg_once(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c]);
+            stg_once(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c]);
         } else {
             const R wn = in.tw[c] + a->beta * hv[c];
-            stg(a->wout + c * fs, pb, wn);
+            stg_once(a->wout + c * fs, pb, wn);
             if (c < 3 && top == 2 && a->pgv) {
                 R *pv = a->pgv + c * a->cstr + k.cxy; // PGV maps share the [nj][nxp] slot size of cov
                 *pv = rmax(rabs(wn), rabs(*pv));
@@ -467,31 +529,39 @@ __global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *p
     }
 }
 
-// Variant "lds": xy-plane staging.  A 256-lane workgroup owns a 64 x 4 tile
-// of (x,y) columns and sweeps a chunk of z planes.  Per plane every lane
-// writes its cell of the 9 stage-input components into an LDS tile that also
-// holds the one-sided halo (1 cell behind and 3 ahead for forward
-// differences, 3 behind and 1 ahead for backward), so all 18 in-plane
-// stencils of a cell are LDS reads at immediate offsets from one address.
-// The z taps stay in the per-lane 5-deep register window; the next plane's
-// global loads (window lead and halo) are issued before the current plane's
-// arithmetic.  Tiles are numbered so that consecutive tiles of one XCD are
-// spatial neighbours (shared halo lines stay in that XCD's L2).
-constexpr int LDS_TY = 4;
+// Variant "lds": xy-plane staging.  A workgroup of LDS_TY wavefronts owns a
+// 64 x LDS_TY tile of (x,y) columns and sweeps a chunk of z planes.  Per plane
+// every lane writes its cell of the 9 stage-input components into an LDS tile
+// that also holds the one-sided halo (1 cell behind and 3 ahead for forward
+// differences, 3 behind and 1 ahead for backward), so all 18 in-plane stencils
+// of a cell are LDS reads at immediate offsets from one address.  The z taps
+// stay in the per-lane 5-deep register window; the next plane's global loads
+// (window lead and halo) are issued before the current plane's arithmetic,
+// unconditionally (lanes outside the grid read a clamped in-bounds address
+// and discard the value), so no branch splits the load stream.  Tiles are
+// numbered so that consecutive tiles of one XCD are spatial neighbours
+// (shared halo lines stay in that XCD's L2).
+#ifndef CGFD_LDS_TY
+#define CGFD_LDS_TY 4
+#endif
+constexpr int LDS_TY = CGFD_LDS_TY;
+static_assert(LDS_TY >= 4 && LDS_TY <= 16 && 4 * LDS_TY <= 64, "x-halo lanes must fit one wavefront");
+// FP64 keeps 4-row tiles: its one wavefront per SIMD needs the full register file
+template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 4; }
 #ifndef CGFD_LDS_WAVES
 #define CGFD_LDS_WAVES 2
 #endif
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
 template <typename R, int SX, int SZ, int ST>
-__global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tindex)
+__global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tindex)
 {
     // The argument block lives in device memory and is read through the
     // constant address space, so fields are fetched with scalar loads where
     // they are used instead of being preloaded into (and spilled from) SGPRs.
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     const AP a0 = (AP)pa;
-    constexpr int TY = LDS_TY, HX = 64 + 4, HY = TY + 4;
+    constexpr int TY = lds_ty<R>(), HX = 64 + 4, HY = TY + 4;
     constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
     constexpr int ZOFF = SZ > 0 ? -1 : -3;
     constexpr int CI = SZ > 0 ? 1 : 3;
@@ -520,20 +590,26 @@ __global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const Stag
     const unsigned esz = sizeof(R);
     const unsigned plb = (unsigned)(g.plane * esz); // bytes per plane
 
-    // halo slots of this lane: one of the 4 halo rows (all 64 columns), and
-    // for lanes 0..15 one of the 4 halo columns (the 4 tile rows)
+    // Addresses.  x < ntx*64 <= nxp always; rows past the last guard row
+    // (y >= nyp, only in a ragged last tile) are clamped onto it (a zero row),
+    // so every load stays inside its own plane.
+    const int yc = min(y, g.nyp - 1);
+    // halo slots of this lane: one of the 4 halo rows (all 64 columns) for
+    // the first 4 wavefronts, and for lanes 0..4*TY-1 one of the 4 halo
+    // columns (the TY tile rows)
     const int hk = tid >> 6;
     const int hrow = SX > 0 ? (hk == 0 ? 0 : TY + hk) : (hk < 3 ? hk : TY + 3);
     const int hy = y0 - O + hrow, hxx = x0 + tx;
-    const bool hv1 = hy >= -3 && hy < g.nj + 3 && hxx < g.ni + 3;
-    const int hc2k = tid >> 2, hr2 = tid & 3;
+    const bool hv1 = hk < 4 && hy >= -3 && hy < g.nyp;
+    const int hc2k = tid / TY, hr2 = tid % TY;
     const int hcol = SX > 0 ? (hc2k == 0 ? 0 : 64 + hc2k) : (hc2k < 3 ? hc2k : 67);
     const int hx2 = x0 - O + hcol, hy2 = y0 + hr2;
-    const bool hv2 = tid < 16 && hx2 >= -3 && hx2 < g.ni + 3 && hy2 < g.nj + 3;
-    // byte offsets inside plane 0 (may be negative: added to a plane base)
-    const int rel_c = (y * g.nxp + x) * (int)esz;
-    const int rel_1 = (hy * g.nxp + hxx) * (int)esz;
-    const int rel_2 = (hy2 * g.nxp + hx2) * (int)esz;
+    const bool hv2 = tid < 4 * TY && hx2 >= -3 && hx2 < g.nxp && hy2 < g.nyp;
+    // byte offsets inside plane 0 (may be negative: added to a plane base);
+    // invalid halo slots fall back to the lane's own cell
+    const int rel_c = (yc * g.nxp + x) * (int)esz;
+    const int rel_1 = hv1 ? (hy * g.nxp + hxx) * (int)esz : rel_c;
+    const int rel_2 = hv2 ? (hy2 * g.nxp + hx2) * (int)esz : rel_c;
     const unsigned base0 = (unsigned)(g.base * esz);
     auto pbase = [&](int zz) { return base0 + (unsigned)zz * plb; };
 
@@ -550,8 +626,8 @@ __global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const Stag
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         lead[c] = ldg(a->win + c * fs0, pbase(z0 + ZOFF + 4) + rel_c);
-        h1[c] = hv1 ? ldg(a->win + c * fs0, pbase(z0) + rel_1) : (R)0;
-        h2[c] = hv2 ? ldg(a->win + c * fs0, pbase(z0) + rel_2) : (R)0;
+        h1[c] = ldg(a->win + c * fs0, pbase(z0) + rel_1);
+        h2[c] = ldg(a->win + c * fs0, pbase(z0) + rel_2);
     }
 
     for (int z = z0; z < z1; ++z) {
@@ -570,18 +646,18 @@ __global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const Stag
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             lds[c][O + ty][O + tx] = w[c][CI];
-            lds[c][hrow][O + tx] = h1[c];
-            if (tid < 16) lds[c][O + hr2][hcol] = h2[c];
+            if (hk < 4) lds[c][hrow][O + tx] = hv1 ? h1[c] : (R)0;
+            if (tid < 4 * TY) lds[c][O + hr2][hcol] = hv2 ? h2[c] : (R)0;
             w[c][4] = lead[c];
         }
         __syncthreads();
-        if (z + 1 < z1) {
+        // next plane's lead and halo: plane z+1+ZOFF+4 <= nk+3 and the halo
+        // plane z+1 <= nk both lie inside the 4 zero planes above the grid
 #pragma unroll
-            for (int c = 0; c < NF; ++c) {
-                lead[c] = ldg(al->win + c * fs, pbase(z + 1 + ZOFF + 4) + rel_c);
-                h1[c] = hv1 ? ldg(al->win + c * fs, pbase(z + 1) + rel_1) : (R)0;
-                h2[c] = hv2 ? ldg(al->win + c * fs, pbase(z + 1) + rel_2) : (R)0;
-            }
+        for (int c = 0; c < NF; ++c) {
+            lead[c] = ldg(al->win + c * fs, pbase(z + 1 + ZOFF + 4) + rel_c);
+            h1[c] = ldg(al->win + c * fs, pbase(z + 1) + rel_1);
+            h2[c] = ldg(al->win + c * fs, pbase(z + 1) + rel_2);
         }
         if (active) {
             R Dx[NF], Dy[NF], Dz[NF];
@@ -632,6 +708,8 @@ __global__ void __launch_bounds__(256) CGFD_LDS_ATTR stage_lds_kernel(const Stag
 #define CGFD_SPLIT_WAVES_T 4
 #endif
 
+constexpr int SPLIT_TY = 4; // rows per workgroup of the split kernels
+
 template <int PART> struct SplitFields;
 template <> struct SplitFields<0> { // stencil inputs: the six stresses
     static constexpr int n = 6;
@@ -649,7 +727,7 @@ stage_split_kernel(const StageArgs<R> *pa, int tindex)
 {
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     constexpr int NIN = SplitFields<PART>::n;
-    constexpr int TY = LDS_TY, HX = 64 + 4, HY = TY + 4;
+    constexpr int TY = SPLIT_TY, HX = 64 + 4, HY = TY + 4;
     constexpr int O = SX > 0 ? 1 : 3;
     constexpr int ZOFF = SZ > 0 ? -1 : -3;
     constexpr int CI = SZ > 0 ? 1 : 3;
@@ -1278,12 +1356,13 @@ static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs
                              int zchunks, hipStream_t s)
 {
     if (variant == 0) {
-        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + LDS_TY - 1) / LDS_TY;
-        stage_lds_kernel<R, SX, SZ, ST><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
+        constexpr int ty = lds_ty<R>();
+        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
+        stage_lds_kernel<R, SX, SZ, ST><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
     } else if (variant == 2) {
-        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + LDS_TY - 1) / LDS_TY;
-        stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
-        stage_split_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, LDS_TY), 0, s>>>(pa, tindex);
+        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
+        stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tindex);
+        stage_split_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tindex);
     } else {
         dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, zchunks);
         stage_direct_kernel<R, SX, SZ, ST><<<grid, dim3(64, rows), 0, s>>>(pa, tindex);

@@ -43,7 +43,11 @@ SIGNATURES = {
     "cgfd_set_pml": (ctypes.c_int, [_VP, _IP, ctypes.POINTER(_VP)]),
     "cgfd_set_forces": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _VP, _VP, ctypes.c_int]),
     "cgfd_set_moments": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _VP, _VP, ctypes.c_int]),
+    "cgfd_set_focal": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _VP, ctypes.c_int, ctypes.c_double]),
     "cgfd_set_receivers": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
+    "cgfd_set_snapshots": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
+    "cgfd_snap_shape": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP]),
+    "cgfd_get_snap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
     "cgfd_set_pgv": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_init": (ctypes.c_int, [_VP]),
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -165,10 +169,37 @@ class Simulation:
         self._check(self.lib.cgfd_set_moments(self.h, n, locs.ctypes.data_as(_IP), _ptr(vec), _ptr(stf),
                                               stf.shape[1]), "set_moments")
 
+    def set_focal(self, locs, series, fdt):
+        """Focal points (the reference's source_type 1): series[6][n][fnt]
+        moment-rate samples (mxx myy mzz mxy mxz myz) every fdt s from t = 0."""
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        ser = self._arr(np.reshape(series, (6, n, -1)))
+        self._check(self.lib.cgfd_set_focal(self.h, n, locs.ctypes.data_as(_IP), _ptr(ser), ser.shape[2],
+                                            float(fdt)), "set_focal")
+
     def set_receivers(self, locs):
         locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
         self.nrecv = locs.shape[0]
         self._check(self.lib.cgfd_set_receivers(self.h, self.nrecv, locs.ctypes.data_as(_IP)), "set_receivers")
+
+    def set_snapshots(self, specs):
+        """specs: one row per snapshot box, xs ys zs xn yn zn xi yi zi tinv cmp
+        (the snap_NNN line of the reference's seismic-output configuration)."""
+        spec = np.ascontiguousarray(specs, dtype=np.int32).reshape(-1, 11)
+        self.nsnap = spec.shape[0]
+        self._check(self.lib.cgfd_set_snapshots(self.h, self.nsnap, spec.ctypes.data_as(_IP)), "set_snapshots")
+
+    def snap_shape(self, isnap):
+        n, t = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.cgfd_snap_shape(self.h, isnap, ctypes.byref(n), ctypes.byref(t)), "snap_shape")
+        return n.value, t.value
+
+    def get_snap(self, isnap, comp):
+        """Component comp of snapshot isnap as [npoints][ntime]."""
+        out = np.empty(self.snap_shape(isnap), dtype=self.dtype)
+        self._check(self.lib.cgfd_get_snap(self.h, isnap, comp, _ptr(out)), "get_snap")
+        return out
 
     def set_pgv(self, on=True):
         self._check(self.lib.cgfd_set_pgv(self.h, 1 if on else 0), "set_pgv")
@@ -185,8 +216,12 @@ class Simulation:
             self.set_forces(sc["force_locs"], sc["force_vec"], sc["force_stf"])
         if sc.get("moment_locs") is not None and len(sc["moment_locs"]):
             self.set_moments(sc["moment_locs"], sc["moment_vec"], sc["moment_stf"])
+        if sc.get("focal_locs") is not None and len(sc["focal_locs"]):
+            self.set_focal(sc["focal_locs"], sc["focal_series"], sc["focal_dt"])
         if sc.get("recv_locs") is not None and len(sc["recv_locs"]):
             self.set_receivers(sc["recv_locs"])
+        if sc.get("snapshots") is not None and len(sc["snapshots"]):
+            self.set_snapshots(sc["snapshots"])
         self.set_pgv(sc.get("pgv", True))
         self.init()
 

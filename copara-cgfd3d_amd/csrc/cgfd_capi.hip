@@ -9,6 +9,7 @@
 #include "cgfd_kernels.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -57,15 +58,31 @@ struct cgfd_ctx {
     int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
     std::vector<int> floc, mloc;
     std::vector<char> fvec, mvec, fstf, mstf;
+    // focal points (source_type 1): locations and [6][n][fnt] series
+    int nfoc = 0, fnt = 0;
+    double fdt = 0;
+    std::vector<int> rmloc;
+    std::vector<char> rmser;
+    int nstf_r = 0;
     int nent = 0;
     int *col_first = nullptr;
     int4 *sent = nullptr;
     char *scoef = nullptr, *fstf_d = nullptr, *mstf_d = nullptr, *fvec_d = nullptr, *mvec_d = nullptr;
+    char *rstf_d = nullptr;
 
     int nrecv = 0;
     std::vector<int> rloc;
     long *rlin = nullptr;
     char *seis = nullptr;
+
+    // snapshot boxes (snap_NNN of seisplot.cpp:350-361)
+    struct Snapshot {
+        int spec[11]; // xs ys zs xn yn zn xi yi zi tinv cmp
+        int npts = 0, ntime = 0, c0 = 0, ncomp = 0;
+        long *lin = nullptr; // padded index of every point, point order k*yn*zn + m*zn + n
+        char *buf = nullptr; // [ntime][ncomp][npts]
+    };
+    std::vector<Snapshot> snaps;
 
     int currT = 0, stage = 0;
     bool inited = false;
@@ -226,6 +243,15 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
     return 0;
 }
 
+static void free_snaps(cgfd_ctx *c)
+{
+    for (auto &sn : c->snaps) {
+        if (sn.lin) (void)hipFree(sn.lin);
+        if (sn.buf) (void)hipFree(sn.buf);
+    }
+    c->snaps.clear();
+}
+
 static void free_sources(cgfd_ctx *c)
 {
     dfree_t(c->col_first);
@@ -235,6 +261,7 @@ static void free_sources(cgfd_ctx *c)
     dfree(c->mstf_d);
     dfree(c->fvec_d);
     dfree(c->mvec_d);
+    dfree(c->rstf_d);
     c->nent = 0;
 }
 
@@ -270,6 +297,7 @@ void cgfd_destroy(cgfd_ctx *c)
     free_sources(c);
     dfree_t(c->rlin);
     dfree(c->seis);
+    free_snaps(c);
     dfree(c->args_d);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -419,6 +447,21 @@ int cgfd_set_moments(cgfd_ctx *c, int n, const int *locs, const void *vec, const
     return 0;
 }
 
+int cgfd_set_focal(cgfd_ctx *c, int n, const int *locs, const void *series, int fnt, double fdt)
+{
+    if (!c || n < 0 || (n > 0 && (!locs || !series || fnt < 1 || !(fdt > 0))))
+        return fail(c, CGFD_EINVAL, "bad focal arguments");
+    int rc = check_points(c, n, locs);
+    if (rc) return rc;
+    c->nfoc = n;
+    c->fnt = n > 0 ? fnt : 0;
+    c->fdt = fdt;
+    c->rmloc.assign(locs, locs + 3 * n);
+    c->rmser.assign((const char *)series, (const char *)series + (size_t)6 * n * c->fnt * c->esz);
+    c->inited = false;
+    return 0;
+}
+
 int cgfd_set_receivers(cgfd_ctx *c, int n, const int *locs)
 {
     if (!c || n < 0 || (n > 0 && !locs)) return fail(c, CGFD_EINVAL, "bad receiver arguments");
@@ -444,6 +487,65 @@ int cgfd_set_receivers(cgfd_ctx *c, int n, const int *locs)
     return 0;
 }
 
+int cgfd_set_snapshots(cgfd_ctx *c, int nsnap, const int *spec)
+{
+    if (!c || nsnap < 0 || (nsnap > 0 && !spec)) return fail(c, CGFD_EINVAL, "bad snapshot arguments");
+    const long NX = c->g.ni + 6, NY = c->g.nj + 6, NZ = c->g.nk + 6;
+    for (int i = 0; i < nsnap; i++) {
+        const int *q = spec + 11 * i;
+        const std::string id = "snapshot " + std::to_string(i + 1) + ": ";
+        if (q[3] < 1 || q[4] < 1 || q[5] < 1 || q[6] < 1 || q[7] < 1 || q[8] < 1)
+            return fail(c, CGFD_EINVAL, id + "counts and intervals must be >= 1");
+        if (q[9] < 1) return fail(c, CGFD_EINVAL, id + "tinv must be >= 1");
+        if (q[10] < 1 || q[10] > 3) return fail(c, CGFD_EINVAL, id + "cmp must be 1 (V), 2 (T) or 3 (both)");
+        // ghost-inclusive locations xs-1+LenFD+k*xi (seisplot.cpp:508-518)
+        const long lo[3] = {q[0] + 2L, q[1] + 2L, q[2] + 2L};
+        const long hi[3] = {lo[0] + (long)(q[3] - 1) * q[6], lo[1] + (long)(q[4] - 1) * q[7],
+                            lo[2] + (long)(q[5] - 1) * q[8]};
+        if (lo[0] < 0 || lo[1] < 0 || lo[2] < 0 || hi[0] >= NX || hi[1] >= NY || hi[2] >= NZ)
+            return fail(c, CGFD_ERANGE, id + "box leaves the grid");
+        if ((long)q[3] * q[4] * q[5] > (1L << 30)) return fail(c, CGFD_EINVAL, id + "too many points");
+    }
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_snaps(c);
+    for (int i = 0; i < nsnap; i++) {
+        cgfd_ctx::Snapshot sn;
+        memcpy(sn.spec, spec + 11 * i, sizeof(sn.spec));
+        const int *q = sn.spec;
+        sn.npts = q[3] * q[4] * q[5];
+        sn.ntime = (c->d.nt + q[9] - 1) / q[9]; // ceil(1.0*nt/tinv)
+        sn.c0 = q[10] == 2 ? TXX : VX;
+        sn.ncomp = q[10] == 1 ? 3 : (q[10] == 2 ? 6 : 9);
+        std::vector<long> lin(sn.npts);
+        for (int k = 0; k < q[3]; k++)
+            for (int m = 0; m < q[4]; m++)
+                for (int n = 0; n < q[5]; n++) {
+                    const long x = q[0] - 1 + k * q[6], y = q[1] - 1 + m * q[7], z = q[2] - 1 + n * q[8];
+                    lin[((size_t)k * q[4] + m) * q[5] + n] = c->g.base + (z * c->g.nyp + y) * c->g.nxp + x;
+                }
+        c->snaps.push_back(sn);
+        cgfd_ctx::Snapshot &dst = c->snaps.back();
+        int rc = dalloc(c, (char **)&dst.lin, lin.size() * sizeof(long));
+        if (!rc) rc = dalloc(c, &dst.buf, (size_t)dst.ntime * dst.ncomp * dst.npts * c->esz);
+        if (rc) {
+            free_snaps(c);
+            return rc;
+        }
+        HIPCHK(c, hipMemcpy(dst.lin, lin.data(), lin.size() * sizeof(long), hipMemcpyHostToDevice));
+    }
+    c->inited = false;
+    return 0;
+}
+
+int cgfd_snap_shape(const cgfd_ctx *c, int isnap, int *npoints, int *ntime)
+{
+    if (!c || isnap < 0 || isnap >= (int)c->snaps.size()) return CGFD_EINVAL;
+    if (npoints) *npoints = c->snaps[isnap].npts;
+    if (ntime) *ntime = c->snaps[isnap].ntime;
+    return 0;
+}
+
 int cgfd_set_pgv(cgfd_ctx *c, int on)
 {
     if (!c) return CGFD_EINVAL;
@@ -452,10 +554,62 @@ int cgfd_set_pgv(cgfd_ctx *c, int on)
     return 0;
 }
 
-// Source entries sorted by (column, z, force before moment, source index), so
-// that a lane meets the sources of its column in z order and applies
-// coincident ones in the reference's order (LoadForce before LoadMoment,
-// each in source-index order).
+} // extern "C"
+
+// ExtractValue (calculate.cu:2586-2613): linear interpolation of a series
+// sampled every DT from Tstart, zero outside [Tstart, Tend], in the
+// reference's Real arithmetic.  `avail` bounds the reads to the flat series
+// array (the series of the following focal points come after this one, as in
+// the reference's H_Rmnt layout).
+template <typename R> static R extract_value(const R *stf, long avail, R Tstart, R Tend, R DT, R time)
+{
+    int PL, PR;
+    R value = 0;
+    if (time > Tend || time < Tstart)
+        value = 0;
+    else {
+        if (std::fabs(time - DT * std::floor(time / DT)) < 1e-5) {
+            PL = (int)(time / DT);
+            PR = PL;
+        } else {
+            PL = (int)(time / DT);
+            PR = PL + 1;
+        }
+        const R vl = PL < avail ? stf[PL] : (R)0, vr = PR < avail ? stf[PR] : (R)0;
+        value = vl + (vr - vl) * ((time - DT * PL) / DT);
+    }
+    return value;
+}
+
+// The moment-rate tensor of every focal point at every stage time, indexed
+// like the source time functions by Tindex = 2*currT + tinc: the stage time
+// (currT + tinc*0.5)*stept (calculate.cu:1422) is a function of Tindex alone.
+// Layout [point][6][2*nt+1]; InterpFocus (calculate.cu:2615-2676) computes
+// the same values on the host once per stage.
+template <typename R> static std::vector<R> focal_table(const cgfd_ctx *c, int nstf)
+{
+    const long all = (long)c->nfoc * c->fnt;
+    const R *ser = (const R *)c->rmser.data();
+    const R stept = (R)c->d.dt, DT = (R)c->fdt;
+    const R Tstart = 0, Tend = DT * (c->fnt - 1);
+    std::vector<R> tab((size_t)c->nfoc * 6 * nstf);
+    for (int t = 0; t < nstf; t++) {
+        const R time = (R)((t * 0.5) * stept);
+        for (int i = 0; i < c->nfoc; i++) {
+            const long src = (long)i * c->fnt;
+            for (int m = 0; m < 6; m++)
+                tab[((size_t)i * 6 + m) * nstf + t] = extract_value<R>(ser + m * all + src, all - src, Tstart, Tend, DT, time);
+        }
+    }
+    return tab;
+}
+
+extern "C" {
+
+// Source entries sorted by (column, z, kind, source index), so that a lane
+// meets the sources of its column in z order and applies coincident ones in
+// the reference's order (LoadForce, LoadMoment, LoadRmom, calculate.cu:
+// 1534-1549, each in source-index order).
 static int build_sources(cgfd_ctx *c)
 {
     free_sources(c);
@@ -467,6 +621,8 @@ static int build_sources(cgfd_ctx *c)
         ents.push_back({(c->floc[3 * i + 1] - 3) * c->g.ni + (c->floc[3 * i] - 3), c->floc[3 * i + 2] - 3, 0, i});
     for (int i = 0; i < c->nmnt; i++)
         ents.push_back({(c->mloc[3 * i + 1] - 3) * c->g.ni + (c->mloc[3 * i] - 3), c->mloc[3 * i + 2] - 3, 1, i});
+    for (int i = 0; i < c->nfoc; i++)
+        ents.push_back({(c->rmloc[3 * i + 1] - 3) * c->g.ni + (c->rmloc[3 * i] - 3), c->rmloc[3 * i + 2] - 3, 2, i});
     if (ents.empty()) return 0;
     std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
         if (a.col != b.col) return a.col < b.col;
@@ -498,6 +654,19 @@ static int build_sources(cgfd_ctx *c)
         HIPCHK(c, hipMemcpy(c->mvec_d, c->mvec.data(), c->mvec.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->mstf_d, c->mstf.data(), c->mstf.size(), hipMemcpyHostToDevice));
     }
+    if (c->nfoc) {
+        c->nstf_r = 2 * c->d.nt + 1;
+        std::vector<char> tab;
+        if (c->prec == CGFD_FP32) {
+            std::vector<float> t = focal_table<float>(c, c->nstf_r);
+            tab.assign((const char *)t.data(), (const char *)(t.data() + t.size()));
+        } else {
+            std::vector<double> t = focal_table<double>(c, c->nstf_r);
+            tab.assign((const char *)t.data(), (const char *)(t.data() + t.size()));
+        }
+        if ((rc = dalloc(c, &c->rstf_d, tab.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->rstf_d, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    }
     c->nent = n;
     if (c->prec == CGFD_FP32)
         launch_src_prep(c->g, (const float *)c->met, (const float *)c->med, c->sent, (const float *)c->fvec_d,
@@ -520,6 +689,8 @@ int cgfd_init(cgfd_ctx *c)
     HIPCHK(c, hipMemsetAsync(c->pgv, 0, (size_t)3 * c->cstr * c->esz, c->stream));
     HIPCHK(c, hipMemsetAsync(c->timg, 0, (size_t)9 * c->tstr * c->esz, c->stream));
     if (c->seis) HIPCHK(c, hipMemsetAsync(c->seis, 0, (size_t)NF * c->nrecv * c->d.nt * c->esz, c->stream));
+    for (auto &sn : c->snaps)
+        HIPCHK(c, hipMemsetAsync(sn.buf, 0, (size_t)sn.ntime * sn.ncomp * sn.npts * c->esz, c->stream));
     if (c->prec == CGFD_FP32)
         launch_velpd(c->g, (const float *)c->met, (const float *)c->med, (const float *)c->pml, (float *)c->cov,
                      c->cstr, c->stream);
@@ -596,8 +767,10 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     sa.scoef = (const R *)c->scoef;
     sa.fstf = (const R *)c->fstf_d;
     sa.mstf = (const R *)c->mstf_d;
+    sa.rstf = (const R *)c->rstf_d;
     sa.nstf_f = c->nstf_f;
     sa.nstf_m = c->nstf_m;
+    sa.nstf_r = c->nstf_r;
     sa.pgv = (k == 3 && c->pgv_on) ? (R *)c->pgv : nullptr;
     sa.h = c->h;
     sa.rh = c->rh;
@@ -685,6 +858,13 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         if (c->currT < c->d.nt && c->nrecv)
             launch_record<R>((const R *)c->set_ptr(SET_FW), c->g.fs, c->rlin, c->nrecv, (R *)c->seis, c->d.nt,
                              c->currT, c->stream);
+        // SWpick after the step (simulation.cc:312-315): every tinv steps
+        for (auto &sn : c->snaps) {
+            const int tinv = sn.spec[9];
+            if (c->currT % tinv != 0 || c->currT / tinv >= sn.ntime) continue;
+            R *slot = (R *)sn.buf + (size_t)(c->currT / tinv) * sn.ncomp * sn.npts;
+            launch_snap<R>((const R *)c->set_ptr(SET_FW), c->g.fs, sn.lin, sn.npts, sn.c0, sn.ncomp, slot, c->stream);
+        }
         HIPCHK(c, hipGetLastError());
         c->currT++;
         c->stage = 0;
@@ -824,6 +1004,25 @@ int cgfd_get_seismo(cgfd_ctx *c, void *out)
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, c->seis, (size_t)NF * c->nrecv * c->d.nt * c->esz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cgfd_get_snap(cgfd_ctx *c, int isnap, int comp, void *out)
+{
+    if (!c || !out || isnap < 0 || isnap >= (int)c->snaps.size()) return fail(c, CGFD_EINVAL, "bad snapshot index");
+    const cgfd_ctx::Snapshot &sn = c->snaps[isnap];
+    if (comp < sn.c0 || comp >= sn.c0 + sn.ncomp)
+        return fail(c, CGFD_EINVAL, "component " + std::to_string(comp) + " is not recorded by this snapshot");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t slab = (size_t)sn.ncomp * sn.npts;
+    std::vector<char> buf(slab * sn.ntime * c->esz);
+    HIPCHK(c, hipMemcpy(buf.data(), sn.buf, buf.size(), hipMemcpyDeviceToHost));
+    const size_t es = c->esz, ci = (size_t)(comp - sn.c0);
+    char *o = (char *)out;
+    for (int t = 0; t < sn.ntime; t++)
+        for (int p = 0; p < sn.npts; p++)
+            memcpy(o + ((size_t)p * sn.ntime + t) * es, buf.data() + ((size_t)t * slab + ci * sn.npts + p) * es, es);
     return 0;
 }
 

@@ -28,11 +28,13 @@ template <typename R> struct StageArgs {
     const R *cov;  // free-surface closure matrices, 18 slots of [nj][nxp]
     long cstr;
     const int *col_first; // per (y*ni+x) column: first source entry or -1
-    const int4 *sent;     // source entries (z, kind, source id, column), sentinel column -1
+    const int4 *sent;     // source entries (z, kind 0 force / 1 moment / 2 focal, source id, column),
+                          // sentinel column -1
     const R *scoef;       // 8 coefficients per entry
     const R *fstf;
     const R *mstf;
-    int nstf_f, nstf_m;
+    const R *rstf; // focal points: interpolated moment rates [point][6][nstf_r]
+    int nstf_f, nstf_m, nstf_r;
     R *pgv; // 3 slots of [nj][nxp] or null
     double h, rh;
     R hr;
@@ -74,5 +76,7 @@ void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent
 template <typename R> void launch_media_prep(const Geom &g, R *med, hipStream_t s);
 template <typename R>
 void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s);
+template <typename R>
+void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot, hipStream_t s);
 
 } // namespace cgfd

@@ -416,7 +416,7 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
         for (int c = VX; c <= VZ; ++c) hv[c] = a->timg[c * a->tstr + tcell];
     }
 
-    // ---- LoadForce then LoadMoment, in source order ------------------------
+    // ---- LoadForce, LoadMoment then LoadRmom, in source order ---------------
     if (e >= 0) {
         int4 en = a->sent[e];
         while (en.w == k.col && en.x == z) {
@@ -427,7 +427,7 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
                 hv[VX] += stf * cf[1] * A;
                 hv[VY] += stf * cf[2] * A;
                 hv[VZ] += stf * cf[3] * A;
-            } else {
+            } else if (en.y == 1) {
                 const R stf = a->mstf[(long)en.z * a->nstf_m + tindex];
                 R A = (R)1;
                 A = stf * A / cf[0];
@@ -437,6 +437,12 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
                 hv[TXY] -= cf[4] * A;
                 hv[TXZ] -= cf[5] * A;
                 hv[TYZ] -= cf[6] * A;
+            } else {
+                // focal point: interpolated moment-rate tensor of this stage
+                const R *fv = a->rstf + (long)en.z * 6 * a->nstf_r + tindex;
+                const R A = cf[0];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) hv[TXX + c] -= fv[(long)c * a->nstf_r] * A;
             }
             en = a->sent[++e];
         }
@@ -1049,6 +1055,11 @@ stage_split_kernel(const StageArgs<R> *pa, int tindex)
                         A = stf * A / cf[0];
 #pragma unroll
                         for (int c = 0; c < 6; ++c) hv[c] -= cf[1 + c] * A;
+                    } else if (PART == 1 && en.y == 2) {
+                        const R *fv = a->rstf + (long)en.z * 6 * a->nstf_r + tindex;
+                        const R A = cf[0];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) hv[c] -= fv[(long)c * a->nstf_r] * A;
                     }
                     en = a->sent[++e];
                 }
@@ -1331,10 +1342,15 @@ __global__ void src_prep_kernel(Geom g, const R *met, const R *med, const int4 *
         if (z == g.nk - 1) A = A * (R)2;
         cf[0] = A;
         for (int k = 0; k < 3; ++k) cf[1 + k] = fvec[3 * en.z + k];
-    } else {
+    } else if (en.y == 1) {
         // calculate.cu:6238-6240: the divisor h*h*h*jac
         cf[0] = hr * hr * hr * jac;
         for (int k = 0; k < 6; ++k) cf[1 + k] = mvec[6 * en.z + k];
+    } else {
+        // LoadRmom, calculate.cu:6314-6316: A = 1/(h*h*h*jac)
+        R A = (R)1;
+        A = A / (hr * hr * hr * jac);
+        cf[0] = A;
     }
 }
 
@@ -1346,6 +1362,17 @@ __global__ void record_kernel(const R *fw, long fs, const long *rlin, int nrecv,
     if (i >= nrecv * NF) return;
     const int r = i / NF, c = i % NF;
     seis[((long)c * nrecv + r) * nt + t] = fw[c * fs + rlin[r]];
+}
+
+// snapshots (SWpick, calculate.cu:2360-2444): ncomp components from c0 of FW
+// at npts points into one time slot laid out [comp][point]
+template <typename R>
+__global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)npts * ncomp) return;
+    const int c = (int)(i / npts), p = (int)(i % npts);
+    slot[i] = fw[(c0 + c) * fs + lin[p]];
 }
 
 // ---------------------------------------------------------------------------
@@ -1432,6 +1459,14 @@ void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent
 }
 
 template <typename R>
+void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot, hipStream_t s)
+{
+    const long n = (long)npts * ncomp;
+    if (n <= 0) return;
+    snap_kernel<R><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(fw, fs, lin, npts, c0, ncomp, slot);
+}
+
+template <typename R>
 void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s)
 {
     if (nrecv <= 0) return;
@@ -1449,6 +1484,7 @@ void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, i
     template void launch_src_prep<R>(const Geom &, const R *, const R *, const int4 *, const R *, const R *,  \
                                      R *, int, R, hipStream_t);                                               \
     template void launch_record<R>(const R *, long, const long *, int, R *, int, int, hipStream_t);       \
+    template void launch_snap<R>(const R *, long, const long *, int, int, int, R *, hipStream_t);          \
     template void launch_media_prep<R>(const Geom &, R *, hipStream_t);
 
 CGFD_INSTANTIATE(float)

@@ -411,13 +411,33 @@ def config_vonkarman(ni=600, nj=600, nk=300, nt=4000, dh=50.0, dt=0.004, npml=20
                 name=f"vonkarman-{ni}x{nj}x{nk}")
 
 
+def focal_series(n, fnt, fdt, rng, m0=1e14):
+    """Moment-rate series of n focal points, [6][n][fnt] float32: a random
+    double couple times a Brune-type rate m0*t/tau^2*exp(-t/tau) starting at
+    a random onset, sampled every fdt from t = 0 (the layout of the
+    reference's focal file, seisnc.cpp:1148-1228)."""
+    t = np.arange(fnt) * fdt
+    ser = np.zeros((6, n, fnt), np.float64)
+    for i in range(n):
+        mech = angle2moment(*rng.uniform([0, 10, -180], [360, 90, 180]))
+        tau, t0 = rng.uniform(3, 8) * fdt, rng.uniform(0, 5) * fdt
+        u = np.clip(t - t0, 0, None)
+        rate = m0 * u / tau ** 2 * np.exp(-u / tau)
+        for c in range(6):
+            ser[c, i] = mech[c] * rate
+    return ser.astype(F32)
+
+
 def config_random(ni, nj, nk, nt, nabs, seed=0, dh=100.0, dt=0.005, topo_amp=300.0, nmom=3, nfrc=2,
-                  nrecv=6, duplicate=True):
+                  nrecv=6, duplicate=True, nfoc=0, fdt=None, fnt=None):
     """Small randomised case for parity tests: curvilinear grid with random
     smooth topography, heterogeneous media, PML on the requested faces,
     moment and force sources (one force on the surface row, and optionally a
     second source on the same cell as the first), receivers on the surface
-    and at depth."""
+    and at depth.  nfoc > 0 adds focal points (moment-rate series every fdt,
+    default 1.5*dt, fnt samples, default covering two thirds of the run so
+    the series also ends inside it); with duplicate, the first focal point
+    shares the cell of the first moment source."""
     rng = np.random.default_rng(seed)
     kx, ky = rng.uniform(0.5, 1.5, 2)
     ph = rng.uniform(0, 2 * np.pi, 2)
@@ -458,10 +478,19 @@ def config_random(ni, nj, nk, nt, nabs, seed=0, dh=100.0, dt=0.005, topo_amp=300
     recv = [[rng.integers(0, ni), rng.integers(0, nj), nk - 1] for _ in range(nrecv // 2)] + \
            [[rng.integers(0, ni), rng.integers(0, nj), rng.integers(0, nk)] for _ in range(nrecv - nrecv // 2)]
     recv = np.array(recv, np.int32) + LFD
-    return dict(ni=ni, nj=nj, nk=nk, dh=dh, dt=dt, nt=nt, metrics=met, alpha=vp, beta=vs, rho=rho,
-                nabs=list(nabs), pml=pml, moment_locs=mloc, moment_vec=mvec, moment_stf=mstf,
-                force_locs=floc, force_vec=fvec, force_stf=fstf, recv_locs=recv, pgv=True,
-                name=f"random-{ni}x{nj}x{nk}-s{seed}")
+    sc = dict(ni=ni, nj=nj, nk=nk, dh=dh, dt=dt, nt=nt, metrics=met, alpha=vp, beta=vs, rho=rho,
+              nabs=list(nabs), pml=pml, moment_locs=mloc, moment_vec=mvec, moment_stf=mstf,
+              force_locs=floc, force_vec=fvec, force_stf=fstf, recv_locs=recv, pgv=True,
+              name=f"random-{ni}x{nj}x{nk}-s{seed}")
+    if nfoc:
+        fdt = float(np.float32(1.5 * dt if fdt is None else fdt))
+        fnt = int(fnt if fnt is not None else max(2, (2 * nt * dt / 3) // fdt))
+        cloc = [[rng.integers(0, ni), rng.integers(0, nj), rng.integers(0, nk)] for _ in range(nfoc)]
+        cloc = np.array(cloc, np.int32).reshape(-1, 3) + LFD
+        if duplicate and nmom:
+            cloc[0] = mloc[0]
+        sc.update(focal_locs=cloc, focal_series=focal_series(nfoc, fnt, fdt, rng), focal_dt=fdt)
+    return sc
 
 
 def random_state(shape, rng, scale_v=1e-3, scale_t=1e5):

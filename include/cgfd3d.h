@@ -78,9 +78,31 @@ int cgfd_set_pml(cgfd_ctx *ctx, const int *nabs, const void *const *tabs);
 int cgfd_set_forces(cgfd_ctx *ctx, int n, const int *locs, const void *vec, const void *stf, int nstf);
 /* ParH2D for momentF (typedata.h:79-86): vec[6n] = mxx myy mzz mxy mxz myz. */
 int cgfd_set_moments(cgfd_ctx *ctx, int n, const int *locs, const void *vec, const void *stf, int nstf);
+/* ParH2D for RmomF (typedata.h:88-97) with the host series of
+ * source_type 1 (source.cpp:43-58): n focal points; series[6][n][fnt] =
+ * mxx myy mzz mxy mxz myz moment-rate samples every fdt seconds from t = 0
+ * (focal_import layout, seisnc.cpp:1148).  Each stage applies their linear
+ * interpolation at the stage time (InterpFocus calculate.cu:2615-2676,
+ * ExtractValue 2586-2613) as LoadRmom does (6257-6328): T -= m(t)/(h^3 J),
+ * after the moment sources. */
+int cgfd_set_focal(cgfd_ctx *ctx, int n, const int *locs, const void *series, int fnt, double fdt);
 /* C2DPointPick (calculate.cu:2678-2746): receivers recording all 9
  * components of the step-end wavefield every step. */
 int cgfd_set_receivers(cgfd_ctx *ctx, int n, const int *locs);
+/* C2DSnapPick (calculate.cu:2230-2358) for the snapshot boxes of
+ * seisplot.cpp:350-361: nsnap boxes, spec[11*nsnap] = xs ys zs (1-based
+ * valid-grid start), xn yn zn (point counts), xi yi zi (index intervals),
+ * tinv, cmp (1 = Vx Vy Vz, 2 = the six stresses, 3 = all nine).  Points sit
+ * at ghost-inclusive (xs+2+k*xi, ys+2+m*yi, zs+2+n*zi) and must lie in the
+ * NX*NY*NZ array.  Replaces any earlier boxes. */
+int cgfd_set_snapshots(cgfd_ctx *ctx, int nsnap, const int *spec);
+/* npoints = xn*yn*zn and ntime = ceil(nt/tinv) of snapshot isnap. */
+int cgfd_snap_shape(const cgfd_ctx *ctx, int isnap, int *npoints, int *ntime);
+/* SWpick (calculate.cu:2360-2444) + SWgather: after every step t with
+ * t % tinv == 0 the step-end field FW is recorded into time slot t/tinv.
+ * Returns component comp (0..8, within the group cmp selects) as
+ * [npoints][ntime], point index k*yn*zn + m*zn + n (seisplot.cpp:513-518). */
+int cgfd_get_snap(cgfd_ctx *ctx, int isnap, int comp, void *out);
 /* output_peakvel (PVF, calculate.cu:134): running max |V| on the surface. */
 int cgfd_set_pgv(cgfd_ctx *ctx, int on);
 

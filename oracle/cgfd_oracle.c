@@ -63,6 +63,10 @@ struct orc {
     int nfrc, nmnt, nstf_f, nstf_m;
     int *floc, *mloc;
     real *fvec, *mvec, *fstf, *mstf;
+    int nfoc, fnt;
+    real fdt;
+    int *rmloc;
+    real *rmser; /* [6][nfoc*fnt]: mxx myy mzz mxy mxz myz series */
     int nrecv;
     int *rloc;
     real *seis;
@@ -155,6 +159,7 @@ void orc_destroy(orc_t *o)
     for (int c = 0; c < NF; c++) { free(o->FW[c]); free(o->W[c]); free(o->tW[c]); free(o->hW[c]); }
     free_aux(o);
     free(o->floc); free(o->mloc); free(o->fvec); free(o->mvec); free(o->fstf); free(o->mstf);
+    free(o->rmloc); free(o->rmser);
     free(o->rloc); free(o->seis);
     free(o);
 }
@@ -232,6 +237,23 @@ int orc_set_moments(orc_t *o, int n, const int *locs, const real *vec, const rea
     if (n > 0 && nstf < 2 * o->nt + 1) return -1;
     if (copy_src(o, n, locs, vec, 6, stf, nstf, &o->mloc, &o->mvec, &o->mstf)) return -1;
     o->nmnt = n; o->nstf_m = nstf;
+    return 0;
+}
+
+int orc_set_focal(orc_t *o, int n, const int *locs, const real *series, int fnt, real fdt)
+{
+    if (n < 0 || (n > 0 && (fnt < 1 || !(fdt > 0)))) return -1;
+    for (int i = 0; i < n; i++) {
+        int x = locs[3 * i], y = locs[3 * i + 1], z = locs[3 * i + 2];
+        if (x < LFD || x >= o->ni + LFD || y < LFD || y >= o->nj + LFD || z < LFD || z >= o->nk + LFD)
+            return -1;
+    }
+    free(o->rmloc); free(o->rmser);
+    o->rmloc = (int *)zalloc((size_t)3 * n * sizeof(int));
+    o->rmser = (real *)zalloc((size_t)6 * n * (n > 0 ? fnt : 1) * sizeof(real));
+    memcpy(o->rmloc, locs, (size_t)3 * n * sizeof(int));
+    if (n > 0) memcpy(o->rmser, series, (size_t)6 * n * fnt * sizeof(real));
+    o->nfoc = n; o->fnt = fnt; o->fdt = fdt;
     return 0;
 }
 
@@ -615,7 +637,40 @@ static void timg(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int
             }
 }
 
-/* ---- LoadForce (calculate.cu:6130-6191) and LoadMoment (6193-6255) -------- */
+#ifdef ORC_DOUBLE
+#define RFLOOR floor
+#define RFABS fabs
+#else
+#define RFLOOR floorf
+#define RFABS fabsf
+#endif
+
+/* ExtractValue (calculate.cu:2586-2613): linear interpolation of one focal
+ * series sampled every DT from Tstart, zero outside [Tstart, Tend]; all in
+ * `real`.  `avail` samples may be read from stf (the flat series of every
+ * later focal point follows, as in the reference's H_Rmnt arrays). */
+static real extract_value(const real *stf, long avail, real Tstart, real Tend, real DT, real time)
+{
+    int PL, PR;
+    real value = 0;
+    if (time > Tend || time < Tstart)
+        value = 0;
+    else {
+        if (RFABS(time - DT * RFLOOR(time / DT)) < 1e-5) {
+            PL = (int)(time / DT);
+            PR = PL;
+        } else {
+            PL = (int)(time / DT);
+            PR = PL + 1;
+        }
+        real vl = PL < avail ? stf[PL] : 0, vr = PR < avail ? stf[PR] : 0;
+        value = vl + (vr - vl) * ((time - DT * PL) / DT);
+    }
+    return value;
+}
+
+/* ---- LoadForce (calculate.cu:6130-6191), LoadMoment (6193-6255) and
+ * LoadRmom (6257-6328) with InterpFocus (2615-2676) ------------------------ */
 static void sources(orc_t *o, int Tindex)
 {
     const int NY = o->NY, NZ = o->NZ;
@@ -644,6 +699,28 @@ static void sources(orc_t *o, int Tindex)
         o->hW[TXY][G] -= o->mvec[6 * mid + 3] * A;
         o->hW[TXZ][G] -= o->mvec[6 * mid + 4] * A;
         o->hW[TYZ][G] -= o->mvec[6 * mid + 5] * A;
+    }
+    if (o->nfoc) {
+        /* stage time (currT + tinc*0.5)*stept, calculate.cu:1422; Tindex = 2*currT+tinc */
+        real time = (Tindex * 0.5) * o->dt;
+        real Tstart = 0, Tend = o->fdt * (o->fnt - 1);
+        const long all = (long)o->nfoc * o->fnt;
+        for (int mid = 0; mid < o->nfoc; mid++) {
+            int idx = o->rmloc[3 * mid], idy = o->rmloc[3 * mid + 1], idz = o->rmloc[3 * mid + 2];
+            long G = ((long)idx * NY + idy) * NZ + idz;
+            long src = (long)mid * o->fnt;
+            real m[6];
+            for (int c = 0; c < 6; c++)
+                m[c] = extract_value(o->rmser + c * all + src, all - src, Tstart, Tend, o->fdt, time);
+            real A = 1.0;
+            A = A / (steph * steph * steph * o->drv[JAC][G]);
+            o->hW[TXX][G] -= m[0] * A;
+            o->hW[TYY][G] -= m[1] * A;
+            o->hW[TZZ][G] -= m[2] * A;
+            o->hW[TXY][G] -= m[3] * A;
+            o->hW[TXZ][G] -= m[4] * A;
+            o->hW[TYZ][G] -= m[5] * A;
+        }
     }
 }
 

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * algorithm (fes478/CoPara-CGFD3D, src/calculate.cu kernels CalDiffCL,
- * CalWaveCL, CalTIMG, VelPDcoeff, LoadForce, LoadMoment, Iteration*, driven by
+ * CalWaveCL, CalTIMG, VelPDcoeff, LoadForce, LoadMoment, LoadRmom, Iteration*, driven by
  * the F/B schedule of src/simulation.cc:252-717).  Only tests, the smoke check
  * and the CPU-baseline leg of the benchmark may load it, and only as the
  * checker.  The product (the HIP library) never calls it.
@@ -48,6 +48,10 @@ int orc_set_pml(orc_t *o, const int *nabs, const real *const *tabs);
  * (mxx myy mzz mxy mxz myz); stf: n*nstf samples at dt/2 */
 int orc_set_forces(orc_t *o, int n, const int *locs, const real *vec, const real *stf, int nstf);
 int orc_set_moments(orc_t *o, int n, const int *locs, const real *vec, const real *stf, int nstf);
+/* focal mechanism (source_type 1): n points; series[6][n][fnt] = mxx myy
+ * mzz mxy mxz myz sampled every fdt from t = 0, interpolated at each stage's
+ * time (InterpFocus, calculate.cu:2615-2676) */
+int orc_set_focal(orc_t *o, int n, const int *locs, const real *series, int fnt, real fdt);
 int orc_set_receivers(orc_t *o, int n, const int *locs);
 int orc_set_pgv(orc_t *o, int on);
 

@@ -38,6 +38,7 @@ def load(double=False):
         "orc_set_pml": (ctypes.c_int, [VP, IP, ctypes.POINTER(VP)]),
         "orc_set_forces": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, VP, ctypes.c_int]),
         "orc_set_moments": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, VP, ctypes.c_int]),
+        "orc_set_focal": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, ctypes.c_int, R]),
         "orc_set_receivers": (ctypes.c_int, [VP, ctypes.c_int, IP]),
         "orc_set_pgv": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_init": (ctypes.c_int, [VP]),
@@ -133,6 +134,15 @@ class Oracle:
         self._check(self.lib.orc_set_moments(self.h, n, locs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                              _ptr(vec), _ptr(stf), stf.shape[1]), "set_moments")
 
+    def set_focal(self, locs, series, fdt):
+        """series: [6][n][fnt] moment-rate samples (mxx myy mzz mxy mxz myz)
+        every fdt seconds from t = 0."""
+        locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
+        n = locs.shape[0]
+        ser = self._arr(np.reshape(series, (6, n, -1)))
+        self._check(self.lib.orc_set_focal(self.h, n, locs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           _ptr(ser), ser.shape[2], fdt), "set_focal")
+
     def set_receivers(self, locs):
         locs = np.ascontiguousarray(locs, dtype=np.int32).reshape(-1, 3)
         self.nrecv = locs.shape[0]
@@ -153,6 +163,8 @@ class Oracle:
             self.set_forces(sc["force_locs"], sc["force_vec"], sc["force_stf"])
         if sc.get("moment_locs") is not None and len(sc["moment_locs"]):
             self.set_moments(sc["moment_locs"], sc["moment_vec"], sc["moment_stf"])
+        if sc.get("focal_locs") is not None and len(sc["focal_locs"]):
+            self.set_focal(sc["focal_locs"], sc["focal_series"], sc["focal_dt"])
         if sc.get("recv_locs") is not None and len(sc["recv_locs"]):
             self.set_receivers(sc["recv_locs"])
         self.set_pgv(sc.get("pgv", True))

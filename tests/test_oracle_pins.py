@@ -239,6 +239,54 @@ def test_source_linearity():
 # ---------------------------------------------------------------------------
 # physics at tolerance level
 # ---------------------------------------------------------------------------
+def test_focal_rate_is_linear_interpolation_of_the_series():
+    """LoadRmom with InterpFocus: with a zero stage input every stencil term
+    vanishes, so the stress rate at a focal cell is exactly -m(t)/(h^3 J) with
+    m(t) the linear interpolation of the series at the stage time
+    (currT + tinc/2)*dt, and zero once t passes the last sample.  numpy's
+    interp states the interpolation independently."""
+    ni, nj, nk, nt = 12, 10, 8, 10
+    sc = homogeneous(ni, nj, nk, nt)
+    # vertical stretching by 1.3 makes J = 1.3, so the 1/J factor is pinned too
+    sc["metrics"] = scenario.metrics_from_coords(
+        *scenario.stretched_grid(ni, nj, nk, DH, None, 1.3 * (nk - 1) * DH), DH)
+    loc = np.array([[6 + 3, 5 + 3, 4 + 3]], np.int32)
+    fdt, fnt = np.float32(0.0075), 6        # 1.5 dt: stage times hit samples every third half step
+    rng = np.random.default_rng(4)
+    ser = (rng.standard_normal((6, 1, fnt)) * 1e12).astype(np.float32)
+    o = oracle.Oracle(ni, nj, nk, DH, DT, nt)
+    o.set_metrics(sc["metrics"])
+    o.set_media(sc["alpha"], sc["beta"], sc["rho"])
+    o.set_pml(sc["nabs"], sc["pml"])
+    o.set_focal(loc, ser, float(fdt))
+    o.init()
+    zero = np.zeros(o.shape, np.float32)
+    x, y, z = loc[0]
+    jac = sc["metrics"]["jac"][x, y, z]
+    assert abs(jac - 1.3) < 1e-5
+    A = np.float32(1.0) / (np.float32(DH) ** 3 * jac)
+    tgrid = np.arange(fnt) * float(fdt)
+    seen_zero = False
+    for step in range(nt):
+        for k, tinc in enumerate((0, 1, 1, 2)):
+            for which in (0, 1, 2):
+                for c in range(9):
+                    o.set_field(c, zero, which)
+            o.run_stages(1)
+            t = float(np.float32((step + tinc * 0.5) * float(np.float32(DT))))
+            for m in range(6):
+                got = o.get_field(3 + m, 3)[x, y, z]
+                if t > tgrid[-1] + 1e-9:
+                    assert got == 0, (step, k, m)
+                    seen_zero = True
+                else:
+                    want = -np.interp(t, tgrid, ser[m, 0].astype(np.float64)) * float(A)
+                    # float32 interpolation: a few ulps of the larger neighbouring sample
+                    tol = 4e-7 * np.abs(ser[m, 0]).max() * float(A)
+                    assert abs(got - want) <= tol, (step, k, m, got, want)
+    assert seen_zero
+
+
 def _peak_time(trace, dt):
     i = int(np.argmax(np.abs(trace)))
     if 0 < i < len(trace) - 1:  # parabolic refinement

@@ -102,6 +102,58 @@ def test_random_scenario_run(variant):
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
+def test_focal_points_run(variant):
+    """Focal points (moment-rate series interpolated at every stage time,
+    LoadRmom) next to moment and force sources, one focal point on the cell
+    of a moment source; the series ends inside the run."""
+    sc = scenario.config_random(44, 38, 26, 48, [5, 4, 4, 5, 5, 0], seed=31, nfoc=4)
+    g, o = _pair(sc, variant=variant)
+    g.run(48)
+    o.run(48)
+    assert_same(g.get_seismo(), o.get_seismo(), "seismograms")
+    _compare_state(g, o, "after 48 steps")
+
+
+def test_snapshots_match_step_end_fields():
+    """Snapshot boxes (SWpick): a full surface plane of velocities every
+    step, a strided interior box of all nine components every 3rd step
+    (ragged: nt is not a multiple of 3), a corner box reaching into the
+    ghost cells (which read zero) with stresses only every 4th step.  The
+    expected values are the oracle's step-end field FW at the same steps, in
+    the reference's point order k*yn*zn + m*zn + n (seisplot.cpp:513-518)."""
+    nt = 13
+    sc = scenario.config_random(30, 26, 18, nt, [4, 4, 3, 3, 4, 0], seed=41, nfoc=1)
+    ni, nj, nk = sc["ni"], sc["nj"], sc["nk"]
+    specs = np.array([[1, 1, nk, ni, nj, 1, 1, 1, 1, 1, 1],
+                      [3, 2, 4, 5, 4, 3, 4, 5, 6, 3, 3],
+                      [-1, -2, 1, 4, 3, 2, 3, 4, 5, 4, 2]], np.int32)
+    sc["snapshots"] = specs
+    g, o = _pair(sc)
+    fields = {}
+    for t in range(nt):
+        o.run(1)
+        fields[t] = [o.get_field(c, 0) for c in range(9)]
+    g.run(nt)
+    for i, q in enumerate(specs):
+        xs, ys, zs, xn, yn, zn, xi, yi, zi, tinv, cmp = (int(v) for v in q)
+        npts, ntime = g.snap_shape(i)
+        assert npts == xn * yn * zn and ntime == -(-nt // tinv)
+        comps = {1: range(0, 3), 2: range(3, 9), 3: range(9)}[cmp]
+        X = xs + 2 + np.arange(xn) * xi
+        Y = ys + 2 + np.arange(yn) * yi
+        Z = zs + 2 + np.arange(zn) * zi
+        for c in comps:
+            want = np.zeros((npts, ntime), np.float32)
+            for slot in range(ntime):
+                f = fields[slot * tinv][c]
+                want[:, slot] = f[np.ix_(X, Y, Z)].reshape(-1)
+            assert_same(g.get_snap(i, c), want, f"snapshot {i} comp {c}")
+        for c in set(range(9)) - set(comps):
+            with pytest.raises(cgfd3d.CgfdError):
+                g.get_snap(i, c)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("nabs", [[0, 0, 0, 0, 0, 0], [7, 0, 0, 3, 5, 0], [3, 3, 3, 3, 3, 2]])
 def test_pml_layouts(nabs, variant):
     sc = scenario.config_random(33, 29, 18, 24, nabs, seed=sum(nabs) + 3)
@@ -163,7 +215,7 @@ def test_flat_config_full_run(variant):
 
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_fp64_build_matches_fp64_oracle(variant):
-    sc = scenario.config_random(40, 30, 24, 30, [5, 4, 4, 5, 5, 0], seed=21)
+    sc = scenario.config_random(40, 30, 24, 30, [5, 4, 4, 5, 5, 0], seed=21, nfoc=2)
     g, o = _pair(sc, double=True, variant=variant)
     g.run(30)
     o.run(30)

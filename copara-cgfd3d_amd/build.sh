@@ -13,6 +13,8 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -ffp-contract=off -fno-gpu-fl
 mkdir -p "$here/_build"
 hipcc "${FLAGS[@]}" ${CGFD_EXTRA_FLAGS:-} -c "$here/csrc/cgfd_kernels.hip" -o "$here/_build/cgfd_kernels.o"
 hipcc "${FLAGS[@]}" -c "$here/csrc/cgfd_capi.hip" -o "$here/_build/cgfd_capi.o"
-hipcc -shared -fPIC --offload-arch="$ARCH" -o "$here/libcgfd3d.so.tmp" "$here/_build/cgfd_kernels.o" "$here/_build/cgfd_capi.o"
+g++ -O2 -std=c++17 -fPIC -ffp-contract=off -Wall -I"$root/include" -c "$here/csrc/cgfd_ncio.cpp" -o "$here/_build/cgfd_ncio.o"
+hipcc -shared -fPIC --offload-arch="$ARCH" -o "$here/libcgfd3d.so.tmp" "$here/_build/cgfd_kernels.o" "$here/_build/cgfd_capi.o" \
+      "$here/_build/cgfd_ncio.o"
 mv -f "$here/libcgfd3d.so.tmp" "$here/libcgfd3d.so"
 echo "built $here/libcgfd3d.so"

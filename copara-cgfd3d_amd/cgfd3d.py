@@ -69,6 +69,15 @@ SIGNATURES = {
     "cgfd_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "cgfd_set_variant": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_stage_bytes": (ctypes.c_double, [_VP]),
+    "cgfd_metrics_specialized": (ctypes.c_int, [_VP]),
+    "cgfd_set_specialize": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_nc_seismo": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP),
+                                      _VP, _IP, _IP, _VP, _VP, _VP]),
+    "cgfd_nc_snap": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _IP, ctypes.c_int, ctypes.POINTER(_VP), _VP]),
+    "cgfd_nc_peakvel": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]),
+    "cgfd_write_seismo": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int, _IP, _IP, _VP, _VP, _VP]),
+    "cgfd_write_snap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_char_p]),
+    "cgfd_write_peakvel": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int]),
     "cgfd_stream": (_VP, [_VP]),
 }
 
@@ -91,6 +100,52 @@ def load_library(path=LIB_PATH):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _prec_of(a):
+    return FP64 if np.asarray(a).dtype == np.float64 else FP32
+
+
+def nc_seismo(path, wave, currtime, pnum, lnum, posx, posy, posz):
+    """seismo.nc from host arrays: wave [9][nrec][npnt], currtime [nrec]."""
+    lib = load_library()
+    dt = np.float64 if _prec_of(wave) == FP64 else np.float32
+    w = np.ascontiguousarray(wave, dtype=dt)
+    nrec, npnt = w.shape[1], w.shape[2]
+    ptrs = (_VP * 9)(*[w[c].ctypes.data for c in range(9)])
+    arrs = [np.ascontiguousarray(a, dtype=dt) for a in (currtime, posx, posy, posz)]
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (pnum, lnum)]
+    rc = lib.cgfd_nc_seismo(os.fsencode(path), _prec_of(wave), nrec, npnt, ptrs, _ptr(arrs[0]),
+                            ints[0].ctypes.data_as(_IP), ints[1].ctypes.data_as(_IP),
+                            _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3]))
+    if rc:
+        raise CgfdError(f"cgfd_nc_seismo failed ({rc})")
+
+
+def nc_snap(path, spec, wave, currtime):
+    """snap_NNN.nc from host arrays: wave maps component -> [nrec][xn][yn][zn]."""
+    lib = load_library()
+    first = next(iter(wave.values()))
+    dt = np.float64 if _prec_of(first) == FP64 else np.float32
+    keep = {c: np.ascontiguousarray(a, dtype=dt) for c, a in wave.items()}
+    ptrs = (_VP * 9)(*[keep[c].ctypes.data if c in keep else None for c in range(9)])
+    sp = np.ascontiguousarray(spec, dtype=np.int32)
+    t = np.ascontiguousarray(currtime, dtype=dt)
+    nrec = t.shape[0]
+    rc = lib.cgfd_nc_snap(os.fsencode(path), _prec_of(first), sp.ctypes.data_as(_IP), nrec, ptrs, _ptr(t))
+    if rc:
+        raise CgfdError(f"cgfd_nc_snap failed ({rc})")
+
+
+def nc_peakvel(path, vx, vy, vz):
+    """peakvel.nc from host [ni][nj] maps."""
+    lib = load_library()
+    dt = np.float64 if _prec_of(vx) == FP64 else np.float32
+    a = [np.ascontiguousarray(v, dtype=dt) for v in (vx, vy, vz)]
+    ni, nj = a[0].shape
+    rc = lib.cgfd_nc_peakvel(os.fsencode(path), _prec_of(vx), ni, nj, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]))
+    if rc:
+        raise CgfdError(f"cgfd_nc_peakvel failed ({rc})")
 
 
 class Simulation:
@@ -299,6 +354,28 @@ class Simulation:
 
     def set_variant(self, variant):
         self._check(self.lib.cgfd_set_variant(self.h, int(variant)), "set_variant")
+
+    # ---- NetCDF outputs (seismo.nc, snap_NNN.nc, peakvel.nc) ---------------
+    def write_seismo(self, path, tinv=1, pnum=None, lnum=None, posx=None, posy=None, posz=None):
+        ints = [None if a is None else np.ascontiguousarray(a, dtype=np.int32) for a in (pnum, lnum)]
+        reals = [None if a is None else self._arr(a) for a in (posx, posy, posz)]
+        self._check(self.lib.cgfd_write_seismo(
+            self.h, os.fsencode(path), int(tinv),
+            *[None if a is None else a.ctypes.data_as(_IP) for a in ints],
+            *[None if a is None else _ptr(a) for a in reals]), "write_seismo")
+
+    def write_snap(self, isnap, path):
+        self._check(self.lib.cgfd_write_snap(self.h, isnap, os.fsencode(path)), "write_snap")
+
+    def write_peakvel(self, path, vz_from_vy=True):
+        self._check(self.lib.cgfd_write_peakvel(self.h, os.fsencode(path), 1 if vz_from_vy else 0), "write_peakvel")
+
+    def metrics_specialized(self):
+        """True when the stage kernel runs with constant horizontal metrics."""
+        return bool(self.lib.cgfd_metrics_specialized(self.h))
+
+    def set_specialize(self, on=True):
+        self._check(self.lib.cgfd_set_specialize(self.h, 1 if on else 0), "set_specialize")
 
     def stage_bytes(self):
         return self.lib.cgfd_stage_bytes(self.h)

@@ -9,6 +9,7 @@
 #include "cgfd_kernels.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,8 @@ struct cgfd_ctx {
     char *pgv = nullptr;
     bool pgv_on = false;
     bool have_metrics = false, have_media = false;
+    bool hmet = false;     // xi/eta metrics are the constants 1/0 in every valid cell
+    bool use_hmet = true;  // let the stage kernel exploit it
 
     // point sources: host copies until init builds the device tables
     int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
@@ -329,6 +332,29 @@ static int download_field(cgfd_ctx *c, const char *src, void *dst)
     return 0;
 }
 
+} // extern "C"
+
+// True when the six xi/eta metrics of every valid cell are bit-exactly
+// xix = etay = 1 and xiy = xiz = etax = etaz = +0 (meshes with fixed x and y
+// spacing; the stage kernel then loads only the zeta metrics).
+template <typename U> static bool horizontal_metrics_const(const cgfd_ctx *c, const void *const *drv, U one)
+{
+    const long NY = c->g.nj + 6, NZ = c->g.nk + 6;
+    const U want[6] = {one, 0, 0, 0, one, 0}; // xix xiy xiz etax etay etaz
+    for (int m = 0; m < 6; m++) {
+        const U *a = (const U *)drv[m];
+        for (long x = 3; x < c->g.ni + 3; x++)
+            for (long y = 3; y < c->g.nj + 3; y++) {
+                const U *col = a + (x * NY + y) * NZ + 3;
+                for (long z = 0; z < c->g.nk; z++)
+                    if (col[z] != want[m]) return false;
+            }
+    }
+    return true;
+}
+
+extern "C" {
+
 int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv)
 {
     if (!c || !drv) return fail(c, CGFD_EINVAL, "null argument");
@@ -338,8 +364,12 @@ int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv)
         int rc = upload_field(c, drv[k], c->met + (size_t)k * c->g.fs * c->esz);
         if (rc) return rc;
     }
+    // compare bit patterns: 1.0 and +0.0 exactly (a -0.0 keeps the general path)
+    c->hmet = c->prec == CGFD_FP32 ? horizontal_metrics_const<uint32_t>(c, drv, 0x3f800000u)
+                                   : horizontal_metrics_const<uint64_t>(c, drv, 0x3ff0000000000000ull);
     c->have_metrics = true;
     c->inited = false;
+    c->args_dirty = true;
     return 0;
 }
 
@@ -778,6 +808,7 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     sa.alpha = k < 3 ? RK4A[k] * dt : (R)0;
     sa.beta = RK4B[k] * dt;
     sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
+    sa.hmet = (c->hmet && c->use_hmet) ? 1 : 0;
     return sa;
 }
 
@@ -1026,6 +1057,109 @@ int cgfd_get_snap(cgfd_ctx *c, int isnap, int comp, void *out)
     return 0;
 }
 
+} // extern "C"
+
+// seismo.nc as seisplot::point_export writes it at the end of a run
+// (seisplot.cpp:672-684 with seisnc.cpp:1336-1365): every step `it` of the
+// run goes to record it/tinv with currTime = (it+1)*stept, so with tinv > 1 a
+// record keeps the last step written into it.
+template <typename R>
+static int write_seismo_t(cgfd_ctx *c, const char *path, int tinv, const int *pnum, const int *lnum, const void *posx,
+                          const void *posy, const void *posz)
+{
+    const int T = c->currT, np = c->nrecv, nt = c->d.nt;
+    const int nrec = T > 0 ? (T - 1) / tinv + 1 : 0;
+    std::vector<R> seis((size_t)NF * np * nt);
+    if (np) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(seis.data(), c->seis, seis.size() * sizeof(R), hipMemcpyDeviceToHost));
+    }
+    std::vector<R> wave((size_t)NF * nrec * np), tcur(nrec);
+    const R stept = (R)c->d.dt;
+    for (int r = 0; r < nrec; r++) {
+        const int it = std::min(r * tinv + tinv - 1, T - 1);
+        tcur[r] = (R)(it + 1) * stept;
+        for (int k = 0; k < NF; k++)
+            for (int p = 0; p < np; p++)
+                wave[((size_t)k * nrec + r) * np + p] = seis[((size_t)k * np + p) * nt + it];
+    }
+    std::vector<int> pn(np), ln(np, 1);
+    std::vector<R> zero(np, (R)0);
+    for (int p = 0; p < np; p++) pn[p] = p;
+    const void *w[NF];
+    for (int k = 0; k < NF; k++) w[k] = wave.data() + (size_t)k * nrec * np;
+    int rc = cgfd_nc_seismo(path, c->prec, nrec, np, w, tcur.data(), pnum ? pnum : pn.data(), lnum ? lnum : ln.data(),
+                            posx ? posx : zero.data(), posy ? posy : zero.data(), posz ? posz : zero.data());
+    return rc ? fail(c, rc, std::string("cannot write ") + path) : 0;
+}
+
+// snap_NNN.nc as seisplot::snap_export writes it (seisplot.cpp:700-760 with
+// seisnc.cpp:1448-1490): ceil(steps/tinv) records, record s holding time slot
+// s with currTime = (s*tinv+1)*stept.
+template <typename R> static int write_snap_t(cgfd_ctx *c, int isnap, const char *path)
+{
+    const cgfd_ctx::Snapshot &sn = c->snaps[isnap];
+    const int tinv = sn.spec[9];
+    const int nrec = std::min((c->currT + tinv - 1) / tinv, sn.ntime);
+    const size_t slab = (size_t)sn.ncomp * sn.npts;
+    std::vector<R> buf(slab * std::max(nrec, 1));
+    if (nrec) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(buf.data(), sn.buf, slab * nrec * sizeof(R), hipMemcpyDeviceToHost));
+    }
+    std::vector<R> wave((size_t)NF * nrec * sn.npts), tcur(nrec);
+    const R stept = (R)c->d.dt;
+    for (int r = 0; r < nrec; r++) {
+        tcur[r] = (R)(r * tinv + 1) * stept;
+        for (int ci = 0; ci < sn.ncomp; ci++)
+            memcpy(wave.data() + ((size_t)(sn.c0 + ci) * nrec + r) * sn.npts, buf.data() + (size_t)r * slab + (size_t)ci * sn.npts,
+                   sn.npts * sizeof(R));
+    }
+    const void *w[NF];
+    for (int k = 0; k < NF; k++) w[k] = wave.data() + (size_t)k * nrec * sn.npts;
+    int rc = cgfd_nc_snap(path, c->prec, sn.spec, nrec, w, tcur.data());
+    return rc ? fail(c, rc, std::string("cannot write ") + path) : 0;
+}
+
+extern "C" {
+
+int cgfd_write_seismo(cgfd_ctx *c, const char *path, int tinv, const int *pnum, const int *lnum, const void *posx,
+                      const void *posy, const void *posz)
+{
+    if (!c || !path || tinv < 1) return fail(c, CGFD_EINVAL, "bad seismogram output arguments");
+    if (!c->nrecv) return fail(c, CGFD_ESTATE, "no receivers are set");
+    HIPCHK(c, hipSetDevice(c->dev));
+    return c->prec == CGFD_FP32 ? write_seismo_t<float>(c, path, tinv, pnum, lnum, posx, posy, posz)
+                                : write_seismo_t<double>(c, path, tinv, pnum, lnum, posx, posy, posz);
+}
+
+int cgfd_write_snap(cgfd_ctx *c, int isnap, const char *path)
+{
+    if (!c || !path || isnap < 0 || isnap >= (int)c->snaps.size()) return fail(c, CGFD_EINVAL, "bad snapshot index");
+    HIPCHK(c, hipSetDevice(c->dev));
+    return c->prec == CGFD_FP32 ? write_snap_t<float>(c, isnap, path) : write_snap_t<double>(c, isnap, path);
+}
+
+int cgfd_write_peakvel(cgfd_ctx *c, const char *path, int vz_from_vy)
+{
+    if (!c || !path) return fail(c, CGFD_EINVAL, "bad peak-velocity output arguments");
+    const long NX = c->g.ni + 6, NY = c->g.nj + 6;
+    std::vector<char> pv((size_t)3 * NX * NY * c->esz), out((size_t)3 * c->g.ni * c->g.nj * c->esz);
+    int rc = cgfd_get_pgv(c, pv.data());
+    if (rc) return rc;
+    const size_t es = c->esz;
+    for (int k = 0; k < 3; k++)
+        for (int x = 0; x < c->g.ni; x++)
+            for (int y = 0; y < c->g.nj; y++)
+                memcpy(out.data() + (((size_t)k * c->g.ni + x) * c->g.nj + y) * es,
+                       pv.data() + (((size_t)k * NX + x + 3) * NY + y + 3) * es, es);
+    const size_t plane = (size_t)c->g.ni * c->g.nj * es;
+    // PV_export (seisnc.cpp:1703-1706) fills Vz from the Vy map
+    rc = cgfd_nc_peakvel(path, c->prec, c->g.ni, c->g.nj, out.data(), out.data() + plane,
+                         out.data() + (vz_from_vy ? 1 : 2) * plane);
+    return rc ? fail(c, rc, std::string("cannot write ") + path) : 0;
+}
+
 // maps stored per valid (x,y) as [slot][nj][nxp] -> reference [slot][NX][NY]
 static int get_xy_maps(cgfd_ctx *c, const char *dev, int nslots, int slot_minor, void *out)
 {
@@ -1110,7 +1244,19 @@ double cgfd_stage_bytes(const cgfd_ctx *c)
     const double npml = (double)c->npml[0] * c->g.nj * c->g.nk + (double)c->npml[1] * c->g.ni * c->g.nk +
                         (double)c->npml[2] * c->g.ni * c->g.nj;
     const double scale = c->esz / 4.0;
-    return scale * (192.0 * nv + 144.0 * npml);
+    // the constant-horizontal-metrics kernel skips 6 of the 9 metric reads
+    const double cell = (c->hmet && c->variant == 0 && c->use_hmet) ? 192.0 - 24.0 : 192.0;
+    return scale * (cell * nv + 144.0 * npml);
+}
+
+int cgfd_metrics_specialized(const cgfd_ctx *c) { return c ? (c->hmet && c->use_hmet && c->variant == 0) : 0; }
+
+int cgfd_set_specialize(cgfd_ctx *c, int on)
+{
+    if (!c) return CGFD_EINVAL;
+    c->use_hmet = on != 0;
+    c->args_dirty = true;
+    return 0;
 }
 
 void *cgfd_stream(const cgfd_ctx *c) { return c ? (void *)c->stream : nullptr; }

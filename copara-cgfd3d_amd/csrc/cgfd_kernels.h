@@ -40,6 +40,7 @@ template <typename R> struct StageArgs {
     R hr;
     R alpha, beta;
     int zlen;
+    int hmet; // 1: horizontal metrics are the constants 1/0 (variant 0 loads only the zeta metrics)
 };
 
 template <typename R> struct TimgArgs {

@@ -148,14 +148,31 @@ template <typename R> struct CellIn {
     R fw[NF], tw[NF];
 };
 
-template <typename R, int ST, typename AP>
+// HM = 1: the horizontal metrics are known to be xi_x = eta_y = 1 and
+// xi_y = xi_z = eta_x = eta_z = +0 in every valid cell (grids with fixed x and
+// y spacing, vertically stretched: the reference's vmap meshes,
+// gridmesh.cpp:101-129, 386-393).  Only the three zeta metrics are loaded;
+// the others enter every product as exact constants, so each operation still
+// rounds (and signs its zeros) exactly as with the loaded values.
+template <typename R, int ST, int HM = 0, typename AP>
 __device__ __forceinline__ void load_cell(AP a, unsigned pb, long fs, CellIn<R> &in)
 {
     in.lam2mu = ldg_once(a->med, pb);
     in.miu = ldg_once(a->med + fs, pb);
     in.rhoinv = ldg_once(a->med + 2 * fs, pb);
+    if (HM) {
+        in.m[XIX] = (R)1;
+        in.m[XIY] = (R)0;
+        in.m[XIZ] = (R)0;
+        in.m[ETX] = (R)0;
+        in.m[ETY] = (R)1;
+        in.m[ETZ] = (R)0;
 #pragma unroll
-    for (int m = 0; m < 9; ++m) in.m[m] = ldg_once(a->met + m * fs, pb);
+        for (int m = ZTX; m <= ZTZ; ++m) in.m[m] = ldg_once(a->met + m * fs, pb);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) in.m[m] = ldg_once(a->met + m * fs, pb);
+    }
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         if (ST == 1 || ST == 2) in.fw[c] = ldg_once(a->fw + c * fs, pb);
@@ -559,7 +576,7 @@ template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 
 #endif
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
-template <typename R, int SX, int SZ, int ST>
+template <typename R, int SX, int SZ, int ST, int HM>
 __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tindex)
 {
     // The argument block lives in device memory and is read through the
@@ -581,9 +598,23 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
     // XCDs; give each XCD a contiguous run of tiles
     const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
     const int nb = gridDim.x, id = blockIdx.x;
+#ifndef CGFD_TILE_MAP
+#define CGFD_TILE_MAP 0
+#endif
+#if CGFD_TILE_MAP == 1
+    const int L = id;
+#else
     const int q = nb / 8, r = nb % 8, xcd = id % 8;
     const int L = xcd * q + min(xcd, r) + id / 8;
+#endif
+#if CGFD_TILE_MAP == 2
+    const int by = L % nty, bx = (L / nty) % ntx, bz = L / (ntx * nty);
+#elif CGFD_TILE_MAP == 3
+    const int nzc = (g.nk + a->zlen - 1) / a->zlen;
+    const int bz = L % nzc, bx = (L / nzc) % ntx, by = L / (nzc * ntx);
+#else
     const int bx = L % ntx, by = (L / ntx) % nty, bz = L / (ntx * nty);
+#endif
     const int x0 = bx * 64, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < g.ni && y < g.nj;
@@ -647,7 +678,7 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
         asm volatile("" : "+s"(al));
         const unsigned pbz = pbase(z) + rel_c;
         CellIn<R> cin;
-        if (active) load_cell<R, ST>(al, pbz, fs, cin);
+        if (active) load_cell<R, ST, HM>(al, pbz, fs, cin);
         __syncthreads(); // the previous plane's LDS reads are done
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
@@ -1385,7 +1416,10 @@ static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs
     if (variant == 0) {
         constexpr int ty = lds_ty<R>();
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
-        stage_lds_kernel<R, SX, SZ, ST><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
+        if (a.hmet)
+            stage_lds_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
+        else
+            stage_lds_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
         stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tindex);

@@ -429,7 +429,7 @@ def focal_series(n, fnt, fdt, rng, m0=1e14):
 
 
 def config_random(ni, nj, nk, nt, nabs, seed=0, dh=100.0, dt=0.005, topo_amp=300.0, nmom=3, nfrc=2,
-                  nrecv=6, duplicate=True, nfoc=0, fdt=None, fnt=None):
+                  nrecv=6, duplicate=True, nfoc=0, fdt=None, fnt=None, shear=True):
     """Small randomised case for parity tests: curvilinear grid with random
     smooth topography, heterogeneous media, PML on the requested faces,
     moment and force sources (one force on the surface row, and optionally a
@@ -437,7 +437,9 @@ def config_random(ni, nj, nk, nt, nabs, seed=0, dh=100.0, dt=0.005, topo_amp=300
     and at depth.  nfoc > 0 adds focal points (moment-rate series every fdt,
     default 1.5*dt, fnt samples, default covering two thirds of the run so
     the series also ends inside it); with duplicate, the first focal point
-    shares the cell of the first moment source."""
+    shares the cell of the first moment source.  shear=False keeps x and y
+    uniform (the reference's vmap meshes), so the xi/eta metrics are exact
+    constants."""
     rng = np.random.default_rng(seed)
     kx, ky = rng.uniform(0.5, 1.5, 2)
     ph = rng.uniform(0, 2 * np.pi, 2)
@@ -446,11 +448,14 @@ def config_random(ni, nj, nk, nt, nabs, seed=0, dh=100.0, dt=0.005, topo_amp=300
     x, y, z = stretched_grid(ni, nj, nk, dh, surf, (nk - 1) * dh)
     # a gentle horizontal shear makes the xi/eta metrics non-trivial too
     NX, NY, NZ = z.shape
-    xx = np.broadcast_to(x, (NX, NY, NZ)).astype(np.float64)
-    xx = (xx + 0.05 * (z.astype(np.float64) + (nk - 1) * dh)).astype(F32)
-    yy = np.broadcast_to(y, (NX, NY, NZ)).astype(np.float64)
-    yy = (yy - 0.03 * xx.astype(np.float64)).astype(F32)
-    met = metrics_from_coords(xx, yy, z, dh)
+    if shear:
+        xx = np.broadcast_to(x, (NX, NY, NZ)).astype(np.float64)
+        xx = (xx + 0.05 * (z.astype(np.float64) + (nk - 1) * dh)).astype(F32)
+        yy = np.broadcast_to(y, (NX, NY, NZ)).astype(np.float64)
+        yy = (yy - 0.03 * xx.astype(np.float64)).astype(F32)
+        met = metrics_from_coords(xx, yy, z, dh)
+    else:
+        met = metrics_from_coords(x, y, z, dh)
     shp = (NX, NY, NZ)
     vp = (5000.0 + 800.0 * rng.standard_normal(shp)).clip(3500, 6500).astype(F32)
     rho = (2500.0 + 200.0 * rng.standard_normal(shp)).clip(2000, 3000).astype(F32)

@@ -142,6 +142,40 @@ int cgfd_get_pgv(cgfd_ctx *ctx, void *out);
 /* matVx2Vz / matVy2Vz of VelPDcoeff (calculate.cu:3320-3428): [NX][NY][9]. */
 int cgfd_get_cov(cgfd_ctx *ctx, void *covx, void *covy);
 
+/* ---- NetCDF output (seisnc.cpp) -------------------------------------------
+ * Files carry the reference's dimensions, variables and global attributes
+ * and are written in the netCDF classic format with 64-bit offsets, which
+ * netCDF readers open like the reference's NETCDF4 files.  Real is float
+ * (CGFD_FP32) or double (CGFD_FP64). */
+/* point_export_def + point_export (seisnc.cpp:1229-1365): seismo.nc with
+ * Vx..Tyz[TimeNum][PointNum], currTime[TimeNum], Pnum, Lnum (int), posx,
+ * posy, posz [PointNum]; wave[9] point to [nrec][npnt] arrays. */
+int cgfd_nc_seismo(const char *path, int precision, int nrec, int npnt, const void *const *wave,
+                   const void *currtime, const int *pnum, const int *lnum, const void *posx, const void *posy,
+                   const void *posz);
+/* snap_export_def + snap_export (seisnc.cpp:1275-1490): snap_NNN.nc with
+ * the components cmp selects as [TimeNum][i][j][k], currTime, and the
+ * IdxStart/IdxNumber/IdxInterval/TimeInterval/Content attributes; spec is
+ * the 11-integer snapshot box of cgfd_set_snapshots; wave[c] (unused
+ * components may be NULL) point to [nrec][xn][yn][zn] arrays. */
+int cgfd_nc_snap(const char *path, int precision, const int *spec, int nrec, const void *const *wave,
+                 const void *currtime);
+/* PV_export (seisnc.cpp:1671-1712): peakvel.nc with Vx, Vy, Vz [i][j]. */
+int cgfd_nc_peakvel(const char *path, int precision, int ni, int nj, const void *vx, const void *vy, const void *vz);
+
+/* End-of-run outputs of a context, as the reference's master writes them
+ * (simulation.cc:319-360).  seismo.nc: every step `it` goes to record
+ * it/tinv (tinv_of_seismo) with currTime = (it+1)*dt, the last step of a
+ * record winning (seisplot.cpp:672-684); pnum, lnum, posx/y/z may be NULL
+ * (point index, line 1, zero positions).  snap_NNN.nc: record s = time slot
+ * s with currTime = (s*tinv+1)*dt (seisplot.cpp:700-760).  peakvel.nc:
+ * vz_from_vy = 1 reproduces the reference's PV_export, which fills Vz from
+ * the Vy map (seisnc.cpp:1703-1706); 0 writes the true Vz map. */
+int cgfd_write_seismo(cgfd_ctx *ctx, const char *path, int tinv, const int *pnum, const int *lnum, const void *posx,
+                      const void *posy, const void *posz);
+int cgfd_write_snap(cgfd_ctx *ctx, int isnap, const char *path);
+int cgfd_write_peakvel(cgfd_ctx *ctx, const char *path, int vz_from_vy);
+
 /* Measurement hooks (no reference counterpart; the reference only prints
  * wall-clock ETA, simulation.cc:710-726).  With profiling on, every stage
  * kernel launch is bracketed by HIP events on the context's stream;
@@ -157,8 +191,16 @@ int cgfd_set_tuning(cgfd_ctx *ctx, int zchunks, int rows);
  * identical results. */
 int cgfd_set_variant(cgfd_ctx *ctx, int variant);
 /* Algorithmic HBM bytes of one RK stage of this configuration (the
- * 192 B/cell + 144 B/PML-cell-axis model, FP32; doubled for FP64). */
+ * 192 B/cell + 144 B/PML-cell-axis model, FP32; doubled for FP64; 168 B/cell
+ * when the constant-horizontal-metrics kernel runs). */
 double cgfd_stage_bytes(const cgfd_ctx *ctx);
+/* 1 when set_metrics found xix = etay = 1 and xiy = xiz = etax = etaz = +0
+ * bit-exactly in every valid cell (fixed x/y spacing, as the reference's vmap
+ * meshes have) and the default stage kernel therefore loads only the three
+ * zeta metrics.  Results are identical either way.  cgfd_set_specialize(ctx,
+ * 0) forces the general kernel. */
+int cgfd_metrics_specialized(const cgfd_ctx *ctx);
+int cgfd_set_specialize(cgfd_ctx *ctx, int on);
 /* The HIP stream (hipStream_t) the context launches on. */
 void *cgfd_stream(const cgfd_ctx *ctx);
 

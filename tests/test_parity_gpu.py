@@ -114,6 +114,30 @@ def test_focal_points_run(variant):
     _compare_state(g, o, "after 48 steps")
 
 
+@pytest.mark.parametrize("seed", [51, 52])
+def test_constant_horizontal_metrics_kernel(seed):
+    """Topography by vertical stretching only (fixed x/y spacing): the
+    default kernel detects the constant xi/eta metrics, loads only the zeta
+    metrics, and still matches the oracle bit for bit; forcing the general
+    kernel gives the same values."""
+    sc = scenario.config_random(41, 35, 22, 30, [5, 4, 4, 5, 5, 0], seed=seed, nfoc=2, shear=False)
+    g, o = _pair(sc)
+    assert g.metrics_specialized()
+    g2 = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    g2.set_specialize(False)
+    g2.setup(sc)
+    assert not g2.metrics_specialized()
+    # 6 of the 9 metric loads (24 B per cell-stage) are gone from the byte model
+    assert g2.stage_bytes() - g.stage_bytes() == 24 * sc["ni"] * sc["nj"] * sc["nk"]
+    g.run(30)
+    g2.run(30)
+    o.run(30)
+    assert_same(g.get_seismo(), o.get_seismo(), "seismograms")
+    _compare_state(g, o, "specialised kernel")
+    for c in range(9):
+        assert_same(g2.get_field(c), g.get_field(c), f"general vs specialised FW[{c}]")
+
+
 def test_snapshots_match_step_end_fields():
     """Snapshot boxes (SWpick): a full surface plane of velocities every
     step, a strided interior box of all nine components every 3rd step

@@ -334,20 +334,22 @@ static int download_field(cgfd_ctx *c, const char *src, void *dst)
 
 } // extern "C"
 
-// True when the six xi/eta metrics of every valid cell are bit-exactly
-// xix = etay = 1 and xiy = xiz = etax = etaz = +0 (meshes with fixed x and y
-// spacing; the stage kernel then loads only the zeta metrics).
-template <typename U> static bool horizontal_metrics_const(const cgfd_ctx *c, const void *const *drv, U one)
+// True when the six xi/eta metrics of every valid cell are xix = etay = 1
+// exactly and xiy = xiz = etax = etaz = 0 (+0 or -0): meshes with fixed x and
+// y spacing, whose cross-product metrics carry zeros of either sign.  The
+// stage kernel then loads only the zeta metrics and multiplies by +0, which
+// can change only the sign of an exactly-zero intermediate, never a value.
+template <typename U> static bool horizontal_metrics_const(const cgfd_ctx *c, const void *const *drv, U one, U sign)
 {
     const long NY = c->g.nj + 6, NZ = c->g.nk + 6;
-    const U want[6] = {one, 0, 0, 0, one, 0}; // xix xiy xiz etax etay etaz
+    const bool is_one[6] = {true, false, false, false, true, false}; // xix xiy xiz etax etay etaz
     for (int m = 0; m < 6; m++) {
         const U *a = (const U *)drv[m];
         for (long x = 3; x < c->g.ni + 3; x++)
             for (long y = 3; y < c->g.nj + 3; y++) {
                 const U *col = a + (x * NY + y) * NZ + 3;
                 for (long z = 0; z < c->g.nk; z++)
-                    if (col[z] != want[m]) return false;
+                    if (is_one[m] ? col[z] != one : (col[z] & ~sign) != 0) return false;
             }
     }
     return true;
@@ -364,9 +366,10 @@ int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv)
         int rc = upload_field(c, drv[k], c->met + (size_t)k * c->g.fs * c->esz);
         if (rc) return rc;
     }
-    // compare bit patterns: 1.0 and +0.0 exactly (a -0.0 keeps the general path)
-    c->hmet = c->prec == CGFD_FP32 ? horizontal_metrics_const<uint32_t>(c, drv, 0x3f800000u)
-                                   : horizontal_metrics_const<uint64_t>(c, drv, 0x3ff0000000000000ull);
+    // compare bit patterns: 1.0 exactly, zeros of either sign
+    c->hmet = c->prec == CGFD_FP32
+                  ? horizontal_metrics_const<uint32_t>(c, drv, 0x3f800000u, 0x80000000u)
+                  : horizontal_metrics_const<uint64_t>(c, drv, 0x3ff0000000000000ull, 0x8000000000000000ull);
     c->have_metrics = true;
     c->inited = false;
     c->args_dirty = true;

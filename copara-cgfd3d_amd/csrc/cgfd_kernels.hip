@@ -149,11 +149,13 @@ template <typename R> struct CellIn {
 };
 
 // HM = 1: the horizontal metrics are known to be xi_x = eta_y = 1 and
-// xi_y = xi_z = eta_x = eta_z = +0 in every valid cell (grids with fixed x and
+// xi_y = xi_z = eta_x = eta_z = 0 in every valid cell (grids with fixed x and
 // y spacing, vertically stretched: the reference's vmap meshes,
 // gridmesh.cpp:101-129, 386-393).  Only the three zeta metrics are loaded;
-// the others enter every product as exact constants, so each operation still
-// rounds (and signs its zeros) exactly as with the loaded values.
+// the others enter every product as exact constants, so each operation
+// rounds exactly as with the loaded values.  A loaded -0 becomes +0, which
+// can flip the sign of an exactly-zero intermediate but no value: nothing on
+// this path divides by, or otherwise distinguishes, a signed zero.
 template <typename R, int ST, int HM = 0, typename AP>
 __device__ __forceinline__ void load_cell(AP a, unsigned pb, long fs, CellIn<R> &in)
 {
@@ -561,9 +563,9 @@ __global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *p
 // stay in the per-lane 5-deep register window; the next plane's global loads
 // (window lead and halo) are issued before the current plane's arithmetic,
 // unconditionally (lanes outside the grid read a clamped in-bounds address
-// and discard the value), so no branch splits the load stream.  Tiles are
-// numbered so that consecutive tiles of one XCD are spatial neighbours
-// (shared halo lines stay in that XCD's L2).
+// and discard the value), so no branch splits the load stream.  A plane's
+// halo is fetched together with its window lead and parked in an LDS ring
+// until the plane is the centre (see below).
 #ifndef CGFD_LDS_TY
 #define CGFD_LDS_TY 4
 #endif
@@ -573,6 +575,9 @@ static_assert(LDS_TY >= 4 && LDS_TY <= 16 && 4 * LDS_TY <= 64, "x-halo lanes mus
 template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 4; }
 #ifndef CGFD_LDS_WAVES
 #define CGFD_LDS_WAVES 2
+#endif
+#ifndef CGFD_HALO_RING
+#define CGFD_HALO_RING 1
 #endif
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
@@ -588,33 +593,51 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
     constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
     constexpr int ZOFF = SZ > 0 ? -1 : -3;
     constexpr int CI = SZ > 0 ? 1 : 3;
-    __shared__ R lds[NF][HY][HX];
+    // The window lead of plane z is plane z+D.  With the halo ring, the halo
+    // rows and columns of a plane are fetched in the same iteration as its
+    // lead, i.e. when the neighbouring tiles fetch those same lines as their
+    // own lead (so they are still in L2), and wait in an LDS ring of D+1
+    // plane tiles until the plane becomes the centre.
+    constexpr int D = ZOFF + 4;
+    constexpr int HD = CGFD_HALO_RING ? D : 0; // halo prefetch distance
+    constexpr int NSLOT = HD + 1;              // 4, 2 or 1 plane tiles
+    __shared__ R lds[NSLOT][NF][HY][HX];
+    auto slot = [](int p) { return p & (NSLOT - 1); };
 
     const AP a = a0;
     const Geom g = geom_of(a);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
 
-    // XCD-aware tile numbering: workgroups are dealt round-robin to the 8
-    // XCDs; give each XCD a contiguous run of tiles
+    // Tile numbering.  Workgroups are dealt round-robin to the 8 XCDs, so
+    // the identity map (default) gives every XCD a uniform sample of the
+    // domain (the same share of PML, surface and source work) and keeps the
+    // running workgroups inside one z chunk.  Map 0 gives each XCD a
+    // contiguous run of tiles (neighbours share its L2, but the XCDs holding
+    // the bottom PML planes finish last); map 4 does that within groups of
+    // 8*CGFD_TILE_GROUP consecutive tiles.  Both cut the L2 fetch traffic by
+    // ~8 % on the 600x600x300 model yet run slower than the identity map.
     const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
     const int nb = gridDim.x, id = blockIdx.x;
 #ifndef CGFD_TILE_MAP
-#define CGFD_TILE_MAP 0
+#define CGFD_TILE_MAP 1
 #endif
 #if CGFD_TILE_MAP == 1
     const int L = id;
+#elif CGFD_TILE_MAP == 4
+    // groups of 8*S consecutive tiles (about what the chip runs at once);
+    // inside a group each XCD takes a contiguous run
+#ifndef CGFD_TILE_GROUP
+#define CGFD_TILE_GROUP 64
+#endif
+    constexpr int G = 8 * CGFD_TILE_GROUP;
+    const int gbase = id / G * G, rem = min(G, nb - gbase), wi = id - gbase;
+    const int q = rem / 8, r = rem % 8, xcd = wi % 8;
+    const int L = gbase + xcd * q + min(xcd, r) + wi / 8;
 #else
     const int q = nb / 8, r = nb % 8, xcd = id % 8;
     const int L = xcd * q + min(xcd, r) + id / 8;
 #endif
-#if CGFD_TILE_MAP == 2
-    const int by = L % nty, bx = (L / nty) % ntx, bz = L / (ntx * nty);
-#elif CGFD_TILE_MAP == 3
-    const int nzc = (g.nk + a->zlen - 1) / a->zlen;
-    const int bz = L % nzc, bx = (L / nzc) % ntx, by = L / (nzc * ntx);
-#else
     const int bx = L % ntx, by = (L / ntx) % nty, bz = L / (ntx * nty);
-#endif
     const int x0 = bx * 64, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < g.ni && y < g.nj;
@@ -659,12 +682,23 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
 #pragma unroll
         for (int c = 0; c < NF; ++c) w[c][j] = ldg(a->win + c * fs0, pbase(z0 + ZOFF + j) + rel_c);
 
+    // halos of the planes z0 .. z0+HD-1 go straight into their ring tiles
+    for (int p = z0; p < z0 + HD; ++p) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+            const R v1 = ldg(a->win + c * fs0, pbase(p) + rel_1);
+            const R v2 = ldg(a->win + c * fs0, pbase(p) + rel_2);
+            if (hk < 4) lds[slot(p)][c][hrow][O + tx] = hv1 ? v1 : (R)0;
+            if (tid < 4 * TY) lds[slot(p)][c][O + hr2][hcol] = hv2 ? v2 : (R)0;
+        }
+    }
+
     R lead[NF], h1[NF], h2[NF];
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
-        lead[c] = ldg(a->win + c * fs0, pbase(z0 + ZOFF + 4) + rel_c);
-        h1[c] = ldg(a->win + c * fs0, pbase(z0) + rel_1);
-        h2[c] = ldg(a->win + c * fs0, pbase(z0) + rel_2);
+        lead[c] = ldg(a->win + c * fs0, pbase(z0 + D) + rel_c);
+        h1[c] = ldg(a->win + c * fs0, pbase(z0 + HD) + rel_1);
+        h2[c] = ldg(a->win + c * fs0, pbase(z0 + HD) + rel_2);
     }
 
     for (int z = z0; z < z1; ++z) {
@@ -679,30 +713,31 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
         const unsigned pbz = pbase(z) + rel_c;
         CellIn<R> cin;
         if (active) load_cell<R, ST, HM>(al, pbz, fs, cin);
-        __syncthreads(); // the previous plane's LDS reads are done
+        __syncthreads(); // the previous plane's LDS reads are done (its tile is slot(z+HD) now)
+        const int sc = slot(z), sh = slot(z + HD);
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
-            lds[c][O + ty][O + tx] = w[c][CI];
-            if (hk < 4) lds[c][hrow][O + tx] = hv1 ? h1[c] : (R)0;
-            if (tid < 4 * TY) lds[c][O + hr2][hcol] = hv2 ? h2[c] : (R)0;
+            lds[sc][c][O + ty][O + tx] = w[c][CI];
+            if (hk < 4) lds[sh][c][hrow][O + tx] = hv1 ? h1[c] : (R)0;
+            if (tid < 4 * TY) lds[sh][c][O + hr2][hcol] = hv2 ? h2[c] : (R)0;
             w[c][4] = lead[c];
         }
         __syncthreads();
-        // next plane's lead and halo: plane z+1+ZOFF+4 <= nk+3 and the halo
-        // plane z+1 <= nk both lie inside the 4 zero planes above the grid
+        // next plane's lead (plane z+1+D <= nk+3) and halo (plane z+1+HD <=
+        // nk+3) lie inside the 4 zero planes above the grid
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
-            lead[c] = ldg(al->win + c * fs, pbase(z + 1 + ZOFF + 4) + rel_c);
-            h1[c] = ldg(al->win + c * fs, pbase(z + 1) + rel_1);
-            h2[c] = ldg(al->win + c * fs, pbase(z + 1) + rel_2);
+            lead[c] = ldg(al->win + c * fs, pbase(z + 1 + D) + rel_c);
+            h1[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_1);
+            h2[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_2);
         }
         if (active) {
             R Dx[NF], Dy[NF], Dz[NF];
 #pragma unroll
             for (int c = 0; c < NF; ++c) {
                 const R ctr = w[c][CI];
-                const R *rx = &lds[c][O + ty][tx]; // taps x-O .. x-O+4
-                const R *ry = &lds[c][ty][O + tx]; // taps y-O .. y-O+4
+                const R *rx = &lds[sc][c][O + ty][tx]; // taps x-O .. x-O+4
+                const R *ry = &lds[sc][c][ty][O + tx]; // taps y-O .. y-O+4
                 if (SX > 0) {
                     Dx[c] = dfd<R, 1>(rx[0], ctr, rx[2], rx[3], rx[4], h, rh);
                     Dy[c] = dfd<R, 1>(ry[0], ctr, ry[2 * HX], ry[3 * HX], ry[4 * HX], h, rh);

@@ -194,11 +194,12 @@ int cgfd_set_variant(cgfd_ctx *ctx, int variant);
  * 192 B/cell + 144 B/PML-cell-axis model, FP32; doubled for FP64; 168 B/cell
  * when the constant-horizontal-metrics kernel runs). */
 double cgfd_stage_bytes(const cgfd_ctx *ctx);
-/* 1 when set_metrics found xix = etay = 1 and xiy = xiz = etax = etaz = +0
- * bit-exactly in every valid cell (fixed x/y spacing, as the reference's vmap
- * meshes have) and the default stage kernel therefore loads only the three
- * zeta metrics.  Results are identical either way.  cgfd_set_specialize(ctx,
- * 0) forces the general kernel. */
+/* 1 when set_metrics found xix = etay = 1 and xiy = xiz = etax = etaz = 0
+ * (either sign) in every valid cell (fixed x/y spacing, as the reference's
+ * vmap meshes have) and the default stage kernel therefore loads only the
+ * three zeta metrics.  Results have identical values either way (only the
+ * sign of an exactly-zero result can differ).  cgfd_set_specialize(ctx, 0)
+ * forces the general kernel. */
 int cgfd_metrics_specialized(const cgfd_ctx *ctx);
 int cgfd_set_specialize(cgfd_ctx *ctx, int on);
 /* The HIP stream (hipStream_t) the context launches on. */

@@ -63,8 +63,10 @@ __device__ __forceinline__ double div_fixed(double s, double h, double rh)
 //   S = +1 (forward):  taps at offsets -1..+3, summed as strF (typenew.h:58)
 //   S = -1 (backward): taps at offsets -3..+1, summed as strB (typenew.h:59),
 //                      i.e. starting from the far (-3) end; the divisor is -h.
-template <typename R, int S>
-__device__ __forceinline__ R dfd(R t0, R t1, R t2, R t3, R t4, double h, double rh)
+// The taps are storage-type values; T may be R itself or R already widened
+// to double (the widening is exact, so the sum is the same).
+template <typename R, int S, typename T>
+__device__ __forceinline__ R dfd(T t0, T t1, T t2, T t3, T t4, double h, double rh)
 {
     double s;
     if (S > 0)

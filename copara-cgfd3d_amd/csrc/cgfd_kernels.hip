@@ -601,7 +601,8 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
     constexpr int D = ZOFF + 4;
     constexpr int HD = CGFD_HALO_RING ? D : 0; // halo prefetch distance
     constexpr int NSLOT = HD + 1;              // 4, 2 or 1 plane tiles
-    __shared__ R lds[NSLOT][NF][HY][HX];
+    typedef R LT;                              // type of the plane tiles
+    __shared__ LT lds[NSLOT][NF][HY][HX];
     auto slot = [](int p) { return p & (NSLOT - 1); };
 
     const AP a = a0;
@@ -623,6 +624,16 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
 #endif
 #if CGFD_TILE_MAP == 1
     const int L = id;
+#elif CGFD_TILE_MAP == 5
+    // whole rows of x tiles on one XCD (x neighbours share its L2), rows
+    // dealt round-robin to the XCDs; the rows past the last multiple of 8
+    // keep the identity map
+    const int nrow = nb / ntx, M = (nrow / 8) * 8 * ntx;
+    int L = id;
+    if (id < M) {
+        const int k = id / 8;
+        L = ((k / ntx) * 8 + id % 8) * ntx + k % ntx;
+    }
 #elif CGFD_TILE_MAP == 4
     // groups of 8*S consecutive tiles (about what the chip runs at once);
     // inside a group each XCD takes a contiguous run
@@ -688,8 +699,8 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
         for (int c = 0; c < NF; ++c) {
             const R v1 = ldg(a->win + c * fs0, pbase(p) + rel_1);
             const R v2 = ldg(a->win + c * fs0, pbase(p) + rel_2);
-            if (hk < 4) lds[slot(p)][c][hrow][O + tx] = hv1 ? v1 : (R)0;
-            if (tid < 4 * TY) lds[slot(p)][c][O + hr2][hcol] = hv2 ? v2 : (R)0;
+            if (hk < 4) lds[slot(p)][c][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
+            if (tid < 4 * TY) lds[slot(p)][c][O + hr2][hcol] = (LT)(hv2 ? v2 : (R)0);
         }
     }
 
@@ -717,9 +728,9 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
         const int sc = slot(z), sh = slot(z + HD);
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
-            lds[sc][c][O + ty][O + tx] = w[c][CI];
-            if (hk < 4) lds[sh][c][hrow][O + tx] = hv1 ? h1[c] : (R)0;
-            if (tid < 4 * TY) lds[sh][c][O + hr2][hcol] = hv2 ? h2[c] : (R)0;
+            lds[sc][c][O + ty][O + tx] = (LT)w[c][CI];
+            if (hk < 4) lds[sh][c][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
+            if (tid < 4 * TY) lds[sh][c][O + hr2][hcol] = (LT)(hv2 ? h2[c] : (R)0);
             w[c][4] = lead[c];
         }
         __syncthreads();
@@ -735,9 +746,9 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
             R Dx[NF], Dy[NF], Dz[NF];
 #pragma unroll
             for (int c = 0; c < NF; ++c) {
-                const R ctr = w[c][CI];
-                const R *rx = &lds[sc][c][O + ty][tx]; // taps x-O .. x-O+4
-                const R *ry = &lds[sc][c][ty][O + tx]; // taps y-O .. y-O+4
+                const LT ctr = (LT)w[c][CI];
+                const LT *rx = &lds[sc][c][O + ty][tx]; // taps x-O .. x-O+4
+                const LT *ry = &lds[sc][c][ty][O + tx]; // taps y-O .. y-O+4
                 if (SX > 0) {
                     Dx[c] = dfd<R, 1>(rx[0], ctr, rx[2], rx[3], rx[4], h, rh);
                     Dy[c] = dfd<R, 1>(ry[0], ctr, ry[2 * HX], ry[3 * HX], ry[4 * HX], h, rh);

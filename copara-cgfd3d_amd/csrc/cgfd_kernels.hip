@@ -1181,7 +1181,7 @@ stage_split_kernel(const StageArgs<R> *pa, int tindex)
 // ---------------------------------------------------------------------------
 // traction image, top three rows (CalTIMG, calculate.cu:5428-5666)
 // ---------------------------------------------------------------------------
-template <typename R, int SX, int SZ>
+template <typename R, int SX, int SZ, int SPLIT>
 __global__ void __launch_bounds__(256) timg_kernel(TimgArgs<R> a)
 {
     const Geom g = a.g;
@@ -1203,14 +1203,23 @@ __global__ void __launch_bounds__(256) timg_kernel(TimgArgs<R> a)
     constexpr int trow[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
     const R *jac = a.met + JAC * fs;
 
+    // The 3 top rows x 3 momentum equations are independent.  Small grids
+    // split them over blockIdx.z (SPLIT = 9: one pair per block; 3: one row
+    // per block) so a thread does not run nine dependent load chains; large
+    // grids keep all nine per thread (SPLIT = 1), which shares the loads of a
+    // column.  The loops stay unrolled, so r and m are compile-time constants
+    // in every copy.
+    const int rz = SPLIT == 9 ? blockIdx.z / 3 : blockIdx.z, mz = blockIdx.z % 3;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
+        if (SPLIT > 1 && r != rz) continue;
         const int z = g.nk - 3 + r;
         const long G = gidx(g, x, y, z);
         const R rhojac = (R)(1.0 / (double)a.med[3 * fs + G] / (double)jac[G]);
         const long tcell = ((long)r * g.nj + y) * row + x;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
+            if (SPLIT == 9 && m != mz) continue;
             const R *T0 = a.win + trow[m][0] * fs;
             const R *T1 = a.win + trow[m][1] * fs;
             const R *T2 = a.win + trow[m][2] * fs;
@@ -1498,14 +1507,24 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
     else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, tindex, rows, zchunks, s);
 }
 
+template <typename R, int SPLIT>
+static void launch_timg_split(int sx, int sz, const TimgArgs<R> &a, dim3 grid, dim3 block, hipStream_t s)
+{
+    if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT><<<grid, block, 0, s>>>(a);
+    else if (sx > 0) timg_kernel<R, 1, -1, SPLIT><<<grid, block, 0, s>>>(a);
+    else if (sz > 0) timg_kernel<R, -1, 1, SPLIT><<<grid, block, 0, s>>>(a);
+    else timg_kernel<R, -1, -1, SPLIT><<<grid, block, 0, s>>>(a);
+}
+
 template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s)
 {
     dim3 block(64, rows);
-    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows);
-    if (sx > 0 && sz > 0) timg_kernel<R, 1, 1><<<grid, block, 0, s>>>(a);
-    else if (sx > 0) timg_kernel<R, 1, -1><<<grid, block, 0, s>>>(a);
-    else if (sz > 0) timg_kernel<R, -1, 1><<<grid, block, 0, s>>>(a);
-    else timg_kernel<R, -1, -1><<<grid, block, 0, s>>>(a);
+    const long cols = (long)a.g.ni * a.g.nj;
+    const int split = cols < 65536 ? 9 : (cols < 131072 ? 3 : 1); // see timg_kernel
+    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, split);
+    if (split == 9) launch_timg_split<R, 9>(sx, sz, a, grid, block, s);
+    else if (split == 3) launch_timg_split<R, 3>(sx, sz, a, grid, block, s);
+    else launch_timg_split<R, 1>(sx, sz, a, grid, block, s);
 }
 
 template <typename R>

@@ -10,9 +10,11 @@ are synthetic (seeded) and resident in HBM before timing starts.
 
 A step is one full RK4 time step (4 stages) of the whole grid.  The metric
 is Gcell-RKstages/s = ni*nj*nk*4*steps / t.  The roofline entry divides the
-algorithmic bytes of one stage (192 B per cell + 144 B per PML cell-axis,
-FP32) by the mean duration of the fused stage kernel measured with HIP events
-on the engine's stream over the timed steps.  The CPU baseline is the oracle
+algorithmic bytes of one stage (192 B per cell, 168 B when the horizontal
+metrics are the constants the mesh implies, + 144 B per PML cell-axis, FP32)
+by the mean duration of the fused stage kernel measured with HIP events on
+the engine's stream over the timed steps; `traffic` is the measured HBM bytes
+per launch from the committed counter summary.  The CPU baseline is the oracle
 (a C/OpenMP restatement of the reference kernels) timed on this host on one
 step of the same problem.
 """
@@ -57,6 +59,62 @@ def describe(sc):
         "storage": "f32",
         "stencil_accumulation": "f64",
     }
+
+
+def batch_bench(args):
+    """Batched FD: `--batch` independent hill-medium scenarios of uneven size
+    (96-192 x 96-192 x 64-128 cells), advanced together with cgfd_run_batch
+    (one stream each, per-grid autotuned z-chunks), against the same
+    scenarios run one after another.  Prints one JSON line."""
+    import cgfd3d
+    import scenario
+
+    nt = args.warmup + args.steps
+    t0 = time.perf_counter()
+    scs = scenario.config_scenario_batch(args.batch, nt_range=(nt, nt), seed=11)
+    log(f"{len(scs)} scenarios built in {time.perf_counter() - t0:.1f} s")
+    sims = []
+    for sc in scs:
+        g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+        g.setup(sc)
+        sims.append(g)
+    zch = [g.autotune(trial_steps=2) for g in sims]
+    cells = sum(sc["ni"] * sc["nj"] * sc["nk"] for sc in scs)
+
+    def timed(run):
+        for g in sims:
+            g.init()
+        run(args.warmup)
+        for g in sims:
+            g.sync()
+        t = time.perf_counter()
+        run(args.steps)
+        for g in sims:
+            g.sync()
+        return time.perf_counter() - t
+
+    def one_by_one(n):
+        for g in sims:  # each scenario finishes before the next starts
+            g.run(n)
+            g.sync()
+
+    t_seq = timed(one_by_one)
+    t_bat = timed(lambda n: cgfd3d.run_batch(sims, n))
+    for g in sims:
+        g.close()
+    value = cells * 4 * args.steps / t_bat / 1e9
+    line = {
+        "metric": "Gcell-RKstages/s", "value": round(value, 4), "unit": "Gcell-RKstages/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_bat / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch-{len(scs)}-hill", "scenarios": [sc["name"] for sc in scs],
+                   "zchunks": zch, "cells_total": cells,
+                   "scenario_steps_per_s_batched": round(len(scs) * args.steps / t_bat, 2),
+                   "scenario_steps_per_s_sequential": round(len(scs) * args.steps / t_seq, 2),
+                   "speedup_vs_sequential": round(t_seq / t_bat, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(sc, steps):
@@ -126,7 +184,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat"])
+    ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat", "batch"])
+    ap.add_argument("--batch", type=int, default=16, help="scenarios of the batch workload")
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
@@ -139,6 +198,10 @@ def main():
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU decomposition is out of scope).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        batch_bench(args)
+        return
 
     import cgfd3d
 

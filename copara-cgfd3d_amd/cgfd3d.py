@@ -53,6 +53,8 @@ SIGNATURES = {
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_sync": (ctypes.c_int, [_VP]),
+    "cgfd_run_batch": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
+    "cgfd_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
     "cgfd_current_step": (ctypes.c_int, [_VP]),
     "cgfd_current_stage": (ctypes.c_int, [_VP]),
     "cgfd_get_field": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
@@ -146,6 +148,18 @@ def nc_peakvel(path, vx, vy, vz):
     rc = lib.cgfd_nc_peakvel(os.fsencode(path), _prec_of(vx), ni, nj, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]))
     if rc:
         raise CgfdError(f"cgfd_nc_peakvel failed ({rc})")
+
+
+def run_batch(sims, nsteps):
+    """Advance several Simulations by nsteps together (one stream each)."""
+    if not sims:
+        return
+    lib = sims[0].lib
+    arr = (_VP * len(sims))(*[s.h for s in sims])
+    rc = lib.cgfd_run_batch(arr, len(sims), int(nsteps))
+    if rc:
+        msgs = "; ".join(lib.cgfd_last_error(s.h).decode() for s in sims)
+        raise CgfdError(f"cgfd_run_batch failed ({rc}): {msgs}")
 
 
 class Simulation:
@@ -283,6 +297,12 @@ class Simulation:
     # ---- stepping ---------------------------------------------------------
     def run(self, nsteps):
         self._check(self.lib.cgfd_run(self.h, int(nsteps)), "run")
+
+    def autotune(self, trial_steps=3):
+        """Pick the fastest z-chunk count for this grid; re-initialises."""
+        z = ctypes.c_int(0)
+        self._check(self.lib.cgfd_autotune(self.h, int(trial_steps), ctypes.byref(z)), "autotune")
+        return z.value
 
     def run_stages(self, nstages):
         self._check(self.lib.cgfd_run_stages(self.h, int(nstages)), "run_stages")

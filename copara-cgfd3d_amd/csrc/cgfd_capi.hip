@@ -936,6 +936,86 @@ int cgfd_run(cgfd_ctx *c, int nsteps)
     return cgfd_run_stages(c, 4 * nsteps);
 }
 
+// Many independent simulations advanced together: stage k of every context
+// is enqueued on that context's own stream before stage k+1 of any, so the
+// GPU overlaps small grids that would leave most CUs idle on their own, and
+// one context's launch gaps are filled by the others' kernels.
+int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
+{
+    if (!ctxs || n < 0 || nsteps < 0) return CGFD_EINVAL;
+    for (int i = 0; i < n; i++) {
+        cgfd_ctx *c = ctxs[i];
+        if (!c) return CGFD_EINVAL;
+        if (!c->inited) return fail(c, CGFD_ESTATE, "cgfd_init must run after the last set_* call");
+        if (c->stage != 0) return fail(c, CGFD_ESTATE, "cgfd_run_batch called in the middle of a step");
+        if ((long)c->currT + nsteps > c->d.nt)
+            return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
+    }
+    for (int t = 0; t < nsteps; t++)
+        for (int k = 0; k < 4; k++)
+            for (int i = 0; i < n; i++) {
+                cgfd_ctx *c = ctxs[i];
+                HIPCHK(c, hipSetDevice(c->dev));
+                int rc = c->prec == CGFD_FP32 ? run_stage<float>(c) : run_stage<double>(c);
+                if (rc) return rc;
+            }
+    return 0;
+}
+
+// Autotuner: time `trial_steps` steps for each candidate z-chunk count
+// around the default and keep the fastest; the state is reset by cgfd_init
+// after every trial and at the end, so the context is as freshly initialised.
+int cgfd_autotune(cgfd_ctx *c, int trial_steps, int *chosen)
+{
+    if (!c || trial_steps < 1) return CGFD_EINVAL;
+    if (!c->inited || c->currT != 0 || c->stage != 0)
+        return fail(c, CGFD_ESTATE, "cgfd_autotune needs a freshly initialised context");
+    HIPCHK(c, hipSetDevice(c->dev));
+    default_tuning(c);
+    // one warm-up step plus the timed ones must fit in the nt the run is sized for
+    trial_steps = std::min(trial_steps, c->d.nt - 1);
+    if (trial_steps < 1) {
+        if (chosen) *chosen = c->zchunks;
+        c->args_dirty = true;
+        return 0;
+    }
+    const int zc0 = c->zchunks, zmax = std::max(1, c->g.nk / 2);
+    std::vector<int> cand = {std::max(1, zc0 / 2), zc0, std::min(zmax, zc0 * 2), std::min(zmax, zc0 * 4)};
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int best = zc0;
+    float best_ms = -1;
+    int rc = 0;
+    for (int zc : cand) {
+        c->zchunks = zc;
+        c->args_dirty = true;
+        if ((rc = cgfd_run_stages(c, 4)) != 0) break; // warm the instantiations
+        if ((rc = hipEventRecord(e0, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
+        if ((rc = cgfd_run_stages(c, 4 * trial_steps)) != 0) break;
+        if ((rc = hipEventRecord(e1, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(c, CGFD_EHIP, "event timing failed");
+            break;
+        }
+        if (best_ms < 0 || ms < best_ms) {
+            best_ms = ms;
+            best = zc;
+        }
+        if ((rc = cgfd_init(c)) != 0) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->zchunks = best;
+    c->args_dirty = true;
+    if (!rc) rc = cgfd_init(c);
+    if (chosen) *chosen = best;
+    return rc;
+}
+
 int cgfd_sync(cgfd_ctx *c)
 {
     if (!c) return CGFD_EINVAL;

@@ -362,6 +362,38 @@ def config_hill(ni=200, nj=200, nk=150, nt=2000, dh=100.0, dt=0.006, npml=12, nr
                 pgv=True, name=f"gaussian-hill-{ni}x{nj}x{nk}")
 
 
+def config_scenario_batch(count, nt_range=(800, 2500), seed=0, size_range=((96, 192), (64, 128)), dh=100.0,
+                          dt=0.006, npml=12, nsta=32, nt_cap=None):
+    """Many independent scenarios on the Gaussian-hill medium (batched FD):
+    grid sizes drawn from U[96,192]^2 x U[64,128], a double couple with random
+    strike/dip/rake at a random interior position 2-10 km deep, log-uniform
+    M0 in [1e14, 1e17], Ricker fc in [1, 2.5] Hz, durations drawn from
+    nt_range steps (nt_cap trims them for short measurements), 32 surface
+    stations per scenario."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        ni, nj = (int(rng.integers(size_range[0][0], size_range[0][1] + 1)) for _ in range(2))
+        nk = int(rng.integers(size_range[1][0], size_range[1][1] + 1))
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        if nt_cap:
+            nt = min(nt, nt_cap)
+        sc = config_hill(ni=ni, nj=nj, nk=nk, nt=nt, dh=dh, dt=dt, npml=npml, nrecv_side=2)
+        depth = rng.uniform(2000.0, min(10000.0, (nk - npml - 4) * dh))
+        x = int(rng.integers(npml + 4, ni - npml - 4))
+        y = int(rng.integers(npml + 4, nj - npml - 4))
+        zk = max(npml + 2, nk - 1 - int(round(depth / dh)))
+        m0 = 10 ** rng.uniform(14.0, 17.0)
+        fc = rng.uniform(1.0, 2.5)
+        sc["moment_locs"] = np.array([[x + LFD, y + LFD, zk + LFD]], np.int32)
+        sc["moment_vec"] = (angle2moment(*rng.uniform([0, 10, -180], [360, 90, 180])) * m0).astype(F32)[None, :]
+        sc["moment_stf"] = stf_table("Ricker", nt, dt, 1.2 / fc, fc)[None, :]
+        sc["recv_locs"] = surface_receivers(ni, nj, nk, 4, max(1, nsta // 4), npml + 2)
+        sc["name"] = f"scenario{i}-{ni}x{nj}x{nk}-nt{nt}"
+        out.append(sc)
+    return out
+
+
 def config_vonkarman(ni=600, nj=600, nk=300, nt=4000, dh=50.0, dt=0.004, npml=20, nsub=200, seed=2024,
                      nrecv_side=20):
     """Large heterogeneous run: random von Karman surface (a=5 km, RMS 300 m),

@@ -118,6 +118,16 @@ int cgfd_run(cgfd_ctx *ctx, int nsteps);
 /* One RKite call (calculate.cu:1355-1599) per stage; the 4th stage of a
  * step finishes it (state copy, traces, step counter). */
 int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
+/* Batch of independent simulations (no reference counterpart; the
+ * reference runs one scenario per job): advance each of the n contexts by
+ * nsteps steps, enqueueing stage k of every context on its own stream before
+ * stage k+1 of any, so small grids share the GPU.  Results equal those of
+ * running each context alone. */
+int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
+/* Autotuner (no reference counterpart): time trial_steps steps for a few
+ * z-chunk counts around the default, keep the fastest (*chosen), and
+ * re-initialise the context.  Call right after cgfd_init. */
+int cgfd_autotune(cgfd_ctx *ctx, int trial_steps, int *chosen);
 int cgfd_sync(cgfd_ctx *ctx);
 int cgfd_current_step(const cgfd_ctx *ctx);
 int cgfd_current_stage(const cgfd_ctx *ctx);

@@ -1,0 +1,790 @@
+"""Parameter-file front end: the inputs of one run from the reference's own
+parameter files.
+
+`load(conf)` reads a SeisFD3D.conf and the files it names (vmap grid,
+absorb, source, station, media) the way the reference's master does
+(fes478/CoPara-CGFD3D src/simulation.cc:759-790, gridmesh.cpp, absorb.cpp,
+source.cpp, seisplot.cpp, mediaparas.cpp) and returns the scenario dict that
+`cgfd3d.Simulation.setup` takes, plus the receiver and snapshot metadata the
+NetCDF writers need.
+
+Arithmetic follows the reference's `Real = float` build:
+- values parsed by `get_conf` are `(float)atof`;
+- anchor blocks are parsed by `fscanf("%f")`;
+- glibc's own `atoi`/`atof`/`strtof` do the parsing;
+- every expression keeps C's float/double promotions;
+- transcendental calls go to the same glibc routines the reference binds:
+  `sinf`/`cosf`/`expf` for float arguments, `sin`/`cos`/`exp` for double ones.
+
+Supported inputs and the reference paths they follow:
+
+| input | reference |
+|---|---|
+| grid_type vmap, hybrid_grid 0 | gridmesh.cpp:72-532, 1092-1210, 1446-1470 |
+| absorb file (ADE CFS-PML) | absorb.cpp:20-46 |
+| source_type 2/3: force and moment anchors, 8 STF types, moment or angle mechanisms | source.cpp:12-155, 671-849, 985-1289 |
+| source_type 1: focal file in netCDF classic format | source.cpp:43-58, 787-848, 1185-1262 |
+| station file: snapshots, receivers, lines | seisplot.cpp:12-55, 330-457 |
+| media_type volume in netCDF classic format | mediaparas.cpp:376-441, 1273-1425, 2140-2219 |
+
+Refused, with the reason in the error:
+- grid_type point: `read_point` stores x, y and z all into `crd.x`
+  (gridmesh.cpp:52-68), so the reference has no valid grid from it.
+- media_type interface: `readdata` overwrites the media from `read_interface`
+  with zero Vp/Vs/rho via `get2in1` (mediaparas.cpp:2163-2200), and
+  `MediaStatistics` then sets every cell to sqrt(0/0).
+- hybrid_grid 1: needs the hybrid Cartesian kernels, which are not part of
+  this engine.
+- NETCDF4/HDF5 media and focal files: no HDF5 reader is available here.
+- media_type interface3D: not restated; pass media arrays instead.
+
+Callers may always pass their own `media=(alpha, beta, rho)` arrays instead.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+import scenario as _sc
+
+F32 = np.float32
+LFD = 3
+PI_F = F32(3.141592653589793)        # constant::PI (typenew.h:111)
+SEIS_ZERO = F32(1e-20)               # constant::SeisZero
+SEIS_EQUAL = F32(1e-5)               # constant::SeisEqual
+SEIS_INF = F32(1e20)                 # constant::SeisInf
+STRLEN = 128                         # constant::SeisStrLen: the fgets buffer
+
+_libc = ctypes.CDLL(None)
+_libc.atoi.restype = ctypes.c_int
+_libc.atoi.argtypes = [ctypes.c_char_p]
+_libc.atof.restype = ctypes.c_double
+_libc.atof.argtypes = [ctypes.c_char_p]
+_libc.strtof.restype = ctypes.c_float
+_libc.strtof.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)]
+_libm = ctypes.CDLL("libm.so.6")
+for _fn in ("sinf", "cosf", "expf"):
+    getattr(_libm, _fn).restype = ctypes.c_float
+    getattr(_libm, _fn).argtypes = [ctypes.c_float]
+
+
+def _sinf(x):
+    return F32(_libm.sinf(float(x)))
+
+
+def _cosf(x):
+    return F32(_libm.cosf(float(x)))
+
+
+def _expf(x):
+    return F32(_libm.expf(float(x)))
+
+
+class ConfError(ValueError):
+    """A parameter file is missing, malformed, or asks for something this
+    front end refuses (the message says which)."""
+
+
+# ---------------------------------------------------------------------------
+# get_conf (common.cpp:23-66, 286-322)
+# ---------------------------------------------------------------------------
+class ConfFile:
+    """One parameter file, searched the way `common::getConfStr` searches it.
+
+    The search reads the file in fgets pieces of at most 127 bytes, skips
+    pieces that start with '#', and takes the first piece whose first token
+    is the key and that has at least `col` tokens; the value is the col-th
+    token (1-based).  `chunk(anchor)` returns the tokens after the anchor
+    line, which `fscanf` reads in the reference."""
+
+    def __init__(self, path, what="configure"):
+        self.path = path
+        try:
+            with open(path, "rb") as f:
+                raw = f.read()
+        except OSError as e:
+            raise ConfError(f"fail to open {what} file {path}: {e.strerror}") from None
+        self.pieces = []
+        lines = raw.split(b"\n")
+        for n, line in enumerate(lines):
+            if n + 1 < len(lines):
+                line += b"\n"
+            for s in range(0, len(line), STRLEN - 1):
+                self.pieces.append(line[s:s + STRLEN - 1])
+
+    def _find(self, key, col):
+        k = key.encode()
+        for n, piece in enumerate(self.pieces):
+            if piece[:1] == b"#":
+                continue
+            toks = piece.split()
+            if toks and toks[0] == k and len(toks) >= col:
+                return toks[col - 1], n
+        raise ConfError(f"there is no value found at {col} colomn for {key} in {self.path}")
+
+    def has(self, key, col=3):
+        try:
+            self._find(key, col)
+            return True
+        except ConfError:
+            return False
+
+    def str(self, key, col=3):
+        return self._find(key, col)[0].decode()
+
+    def int(self, key, col=3):
+        return int(_libc.atoi(self._find(key, col)[0]))
+
+    def real(self, key, col=3):
+        """(float)atof of the token, as get_conf(..., float *) stores it."""
+        return F32(_libc.atof(self._find(key, col)[0]))
+
+    def bool(self, key, col=3):
+        return self._find(key, col)[0] != b"0"
+
+    def chunk(self, anchor):
+        _, n = self._find(anchor, 1)
+        toks = []
+        for piece in self.pieces[n + 1:]:
+            toks.extend(piece.split())
+        return _Scanner(toks, f"{self.path} after {anchor}")
+
+
+class _Scanner:
+    """fscanf("%f") over a token stream: strtof of each token."""
+
+    def __init__(self, toks, where):
+        self.toks, self.pos, self.where = toks, 0, where
+
+    def real(self):
+        if self.pos >= len(self.toks):
+            raise ConfError(f"{self.where}: fewer values than expected")
+        tok = self.toks[self.pos]
+        end = ctypes.c_char_p()
+        buf = ctypes.create_string_buffer(tok)
+        v = _libc.strtof(buf, ctypes.byref(end))
+        if ctypes.cast(end, ctypes.c_void_p).value == ctypes.addressof(buf):
+            raise ConfError(f"{self.where}: '{tok.decode()}' is not a number")
+        self.pos += 1
+        return F32(v)
+
+    def reals(self, n):
+        return np.array([self.real() for _ in range(n)], F32)
+
+
+# ---------------------------------------------------------------------------
+# main parameter file (simulation.cc:759-790 and the getConf of each part)
+# ---------------------------------------------------------------------------
+def read_main(conf_path, seispath=None):
+    """Scalars of SeisFD3D.conf and the paths of the files it names.
+    `seispath` overrides the file's own `seispath` (the directory that holds
+    the other files)."""
+    c = ConfFile(conf_path, "main par")
+    m = {
+        "ni": c.int("ni"), "nj": c.int("nj"), "nk": c.int("nk"), "nt": c.int("nt"),
+        "stept": c.real("stept"), "steph": c.real("steph"),
+        "seispath": seispath if seispath is not None else c.str("seispath"),
+        "grid_type": c.str("grid_type"), "hybrid_grid": c.int("hybrid_grid"),
+        "media_type": c.str("media_type"),
+        "sampling": [c.int("sampling_point_per_cell", 3 + i) for i in range(3)],
+        "output_peakvel": c.int("output_peakvel"),
+    }
+    for key in ("grid_filename", "media_filename", "source_filename", "station_filename", "absorb_filename"):
+        m[key] = f"{m['seispath']}/{c.str(key)}"  # sprintf("%s/%s", parpath, name)
+    if m["ni"] <= 0 or m["nj"] <= 0 or m["nk"] <= 0 or m["nt"] <= 0:
+        raise ConfError(f"{conf_path}: ni, nj, nk and nt must be positive")
+    if max(m["sampling"]) > 1:
+        raise ConfError("sampling_point_per_cell above 1 asks for the media smoothing of ApplySmooth "
+                        "(mediaparas.cpp:1712-1824), which this front end does not restate")
+    return m
+
+
+# ---------------------------------------------------------------------------
+# vmap grid (gridmesh.cpp:72-532 with hybrid_grid = 0; 1092-1210)
+# ---------------------------------------------------------------------------
+def _set_twoside_space(l, na, nb, ha1, ha2, hb1, hb2):
+    """gridmesh::set_twoside_space (gridmesh.cpp:1184-1210), float arithmetic."""
+    l0 = F32(F32(l - F32(F32(na) * ha1)) - F32(F32(nb) * hb1))
+    la = F32(F32(na) * F32(ha2 - ha1))
+    lb = F32(F32(nb) * F32(hb2 - hb1))
+    if abs(l0) < SEIS_EQUAL:
+        return ha1, hb1
+    if abs(la) < SEIS_EQUAL:
+        return ha1, F32(hb1 + F32(l0 / F32(nb)))
+    if abs(lb) < SEIS_EQUAL:
+        return F32(ha1 + F32(l0 / F32(na))), hb1
+    return (F32(ha1 + F32(F32(F32(l0 * la) / F32(la + lb)) / F32(na))),
+            F32(hb1 + F32(F32(F32(l0 * lb) / F32(la + lb)) / F32(nb))))
+
+
+def smooth_zdepth(za, zb, ha, hb, n):
+    """gridmesh::smooth_zdepth (gridmesh.cpp:1092-1182): n+1 depths from zb
+    (bottom) to about za whose spacing changes from hb at the bottom to ha
+    at the top; float arithmetic with the source's double promotions."""
+    za, zb, ha, hb = F32(za), F32(zb), F32(ha), F32(hb)
+    l = F32(za - zb)
+    h0 = F32(l / F32(n))
+    h = [F32(0.0)] * (n + 2)
+    h[0], h[n + 1] = hb, ha
+    if (h0 >= ha and h0 <= hb) or (h0 <= ha and h0 >= hb):
+        nb, na, n0 = 1, n, n
+        for _ in range(4 * n + 8):
+            l0 = F32(0.0)
+            f = F32(math.pow(float(F32(ha / hb)), 1.0 / (n0 + 1)) - 1.0)
+            for k in range(1, nb):
+                h[k] = hb
+            for k in range(na + 1, n + 2):
+                h[k] = ha
+            for k in range(nb, na + 1):
+                h[k] = F32(h[k - 1] * F32(F32(1.0) + f))
+                l0 = F32(l0 + h[k])
+            ha1 = F32(float(ha) * min(1.0, 1.0 - float(f)))
+            ha2 = F32(float(ha) * max(1.0, 1.0 - float(f)))
+            hb1 = F32(float(hb) * min(1.0, 1.0 + float(f)))
+            hb2 = F32(float(hb) * max(1.0, 1.0 + float(f)))
+            la1, la2 = F32(F32(n - na) * ha1), F32(F32(n - na) * ha2)
+            lb1, lb2 = F32(F32(nb - 1) * hb1), F32(F32(nb - 1) * hb2)
+            rest = F32(l - l0)
+            if F32(la1 + lb1) <= rest <= F32(la2 + lb2):
+                ha0, hb0 = _set_twoside_space(rest, n - na, nb - 1, ha1, ha2, hb1, hb2)
+                for k in range(na + 1, n + 1):
+                    h[k] = ha0
+                for k in range(1, nb):
+                    h[k] = hb0
+                break
+            if rest < F32(la1 + lb1):
+                if ha < hb:
+                    na, n0 = na - 1, n0 - 1
+                else:
+                    nb, n0 = nb + 1, n0 - 1
+            else:
+                if ha < hb:
+                    nb, n0 = nb + 1, n0 - 1
+                else:
+                    na, n0 = na - 1, n0 - 1
+            if n0 < 0 or na < nb - 1:
+                break
+        else:
+            raise ConfError("smooth_zdepth: no transition spacing fits this layer")
+        if n0 < 0 or na < nb - 1:
+            raise ConfError("smooth_zdepth: no transition spacing fits this layer")
+    else:
+        na = nb = n // 4
+        n0 = n - na - nb
+        la1 = F32((na + 1) * na / 2.0)
+        lb1 = F32((nb + 1) * nb / 2.0)
+        la2 = F32(F32(la1 + F32(F32(F32(na) * lb1) / F32(nb))) + F32(na * n0))
+        t = F32(l - F32(F32(na) * ha))
+        t = F32(t - F32(F32(nb) * hb))
+        t = F32(t - F32(F32(n0) * ha))
+        l0 = F32(t - F32(F32(la1 * F32(ha - hb)) / F32(nb)))
+        ha0 = F32(l0 / la2)
+        hb0 = F32(F32(F32(F32(na) * ha0) + ha - hb) / F32(nb))
+        for k in range(1, nb + 1):
+            h[k] = F32(hb + F32(F32(k) * hb0))
+        for k in range(nb + 1, nb + n0 + 1):
+            h[k] = h[k - 1]
+        for k in range(nb + n0 + 1, n + 1):
+            h[k] = F32(ha + F32(F32(n - k + 1) * ha0))
+    z = [F32(0.0)] * (n + 1)
+    for k in range(n):
+        z[k + 1] = F32(z[k] + h[k + 1])
+    return np.array([F32(v + zb) for v in z], F32)
+
+
+def read_vmap(path, ni, nj, nk):
+    """Anchors of a vmap grid file: (posx[ni][nj], posy[ni][nj],
+    posz[ni][nj][nsurf], gridpoints, equalspacing), with the checks of
+    read_vmapnew (gridmesh.cpp:72-156)."""
+    c = ConfFile(path, "gridvmap data")
+    dims = [c.int("vmap_dims", 3 + i) for i in range(3)]
+    ns = dims[2]
+    if ns < 2:
+        raise ConfError(f"{path}: vmap_dims needs at least 2 surfaces")
+    gp = [c.int("vmap_gridpoints", 3 + i) for i in range(ns - 1)]
+    eq = [c.bool("vmap_equalspacing", 3 + i) for i in range(ns - 1)]
+    s = c.chunk("<vmap_anchor>")
+    vals = s.reals(dims[0] * dims[1] * (ns + 2)).reshape(dims[0], dims[1], ns + 2)
+    posx, posy, posz = vals[:, :, 0].copy(), vals[:, :, 1].copy(), vals[:, :, 2:].copy()
+    if dims[0] != ni:
+        raise ConfError("The X value number(dims[0]) obtained from gridfile is different with configure number(cdx.ni)")
+    if dims[1] != nj:
+        raise ConfError("The Y value number(dims[0]) obtained from gridfile is different with configure number(cdx.nj)")
+    if not eq[0] or not eq[ns - 2]:
+        raise ConfError("The first layer and last layer should be equal spacing")
+    for i in range(1, ns - 1):
+        if not eq[i - 1] and not eq[i]:
+            raise ConfError("There is only one layer(in two adjacent layers) need to adjust the grid spacing")
+    if sum(gp) + 1 != nk:
+        raise ConfError("The Z value number[ sum(gridpoints)+1 ] obtained from gridfile is different with "
+                        "configure number(cdx.nk)")
+    bad = np.argwhere(posz[:, :, 1:] > posz[:, :, :-1])
+    if len(bad):
+        i, j, k = bad[0]
+        raise ConfError(f"The Z direction depth should be monotonically decreasing, while position[{i}][{j}][{k + 1}]"
+                        f" should deeper than [{i}][{j}][{k}]")
+    return posx, posy, posz, gp, eq
+
+
+def vmap_coords(posx, posy, posz, gp, eq, nk):
+    """Coordinates with ghosts (x, y, z each (ni+6, nj+6, nk+6)) from vmap
+    anchors: the all-curvilinear branch of read_vmapnew
+    (gridmesh.cpp:439-513) and the ghost extrapolation of readdata
+    (gridmesh.cpp:1467-1469)."""
+    ni, nj, ns = posz.shape
+    shape = (ni + 2 * LFD, nj + 2 * LFD, nk + 2 * LFD)
+    x = np.zeros(shape, F32)
+    y = np.zeros(shape, F32)
+    z = np.zeros(shape, F32)
+    v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj))
+    x[v] = posx[:, :, None]
+    y[v] = posy[:, :, None]
+    zindex = [0] * ns
+    zindex[ns - 1] = LFD
+    for i in range(ns - 2, -1, -1):
+        zindex[i] = zindex[i + 1] + gp[i]
+    zvec = np.zeros((ni, nj, nk + 2 * LFD), F32)
+    for k in range(ns - 1):
+        hi, lo = zindex[k], zindex[k + 1]
+        if eq[k]:
+            step = ((posz[:, :, k] - posz[:, :, k + 1]).astype(np.float64) / (1.0 * gp[k])).astype(F32)
+            for m in range(lo, hi + 1):
+                zvec[:, :, m] = posz[:, :, k] + F32(m - hi) * step
+        else:
+            for i in range(ni):
+                for j in range(nj):
+                    p = posz[i, j]
+                    zs0 = F32(float(F32(p[k - 1] - p[k])) / (1.0 * gp[k - 1]))
+                    zs1 = F32(float(F32(p[k + 1] - p[k + 2])) / (1.0 * gp[k + 1]))
+                    zvec[i, j, lo:lo + gp[k] + 1] = smooth_zdepth(p[k], p[k + 1], zs0, zs1, gp[k])
+        zvec[:, :, hi] = posz[:, :, k]
+        zvec[:, :, lo] = posz[:, :, k + 1]
+    z[v + (slice(LFD, LFD + nk),)] = zvec[:, :, LFD:LFD + nk]
+    for a in (x, y, z):
+        _sc.interpolated_extend(a)
+    return x, y, z
+
+
+# ---------------------------------------------------------------------------
+# absorbing layers (absorb.cpp:20-46)
+# ---------------------------------------------------------------------------
+def read_absorb(path):
+    c = ConfFile(path, "absorb configure")
+    nabs = [c.int("abs_number", 3 + i) for i in range(6)]
+    vel = [c.real("abs_velocity", 3 + i) for i in range(6)]
+    bmax = [c.real("bmax", 3 + i) for i in range(6)]
+    amax = [c.real("amax", 3 + i) for i in range(6)]
+    return nabs, vel, bmax, amax
+
+
+# ---------------------------------------------------------------------------
+# source time functions (mathfunc.cpp:481-610), float build
+# ---------------------------------------------------------------------------
+_SQRT_PI_F = F32(np.sqrt(PI_F))      # sqrt(PI) binds sqrtf: PI is a float
+
+
+def _stf_value(name, x, t0, f0):
+    """One sample of a source time function as cal_stf_force/cal_stf_moment
+    call it (source.cpp:75-96): x = i*dt - start; Bell-type functions get
+    x - t0 and width f0."""
+    x, t0, f0 = F32(x), F32(t0), F32(f0)
+    if name in ("Gauss", "gauss"):
+        if abs(t0) >= SEIS_ZERO and (x <= 0.0 or float(x) >= 2.0 * float(t0)):
+            return F32(0.0)
+        d = F32(x - t0)
+        g = F32(_expf(F32(-F32(F32(d * d) / F32(f0 * f0)))) / F32(_SQRT_PI_F * f0))
+        if name == "Gauss":
+            return g
+        return F32(float(g) * (-2.0 * float(d) / float(F32(f0 * f0))))
+    if name in ("Ricker", "ricker"):
+        if x <= 0.0:
+            return F32(0.0)
+        u = F32(float(F32(x - t0)) * 2.0 * float(PI_F) * float(f0))
+        uu = float(F32(u * u))
+        e = math.exp(-uu / 4.0)
+        if name == "Ricker":
+            return F32((uu / 4.0 - 0.5) * e * float(_SQRT_PI_F) / 2.0)
+        return F32(float(u) * (1.5 - uu / 4.0) * e * float(_SQRT_PI_F) / 2.0 * float(PI_F) * float(f0))
+    if name in ("Bell", "bell", "BELL"):
+        xb = F32(x - t0)
+        w = 2.0 * float(PI_F) * float(xb) / float(f0)
+        if name == "Bell":
+            return F32((1.0 - math.cos(w)) / float(f0)) if 0.0 < xb < f0 else F32(0.0)
+        if name == "bell":
+            return F32(2.0 * float(PI_F) * math.sin(w) / float(f0)) if 0.0 < xb < f0 else F32(0.0)
+        if xb <= 0.0:
+            return F32(0.0)
+        if xb < f0:
+            return F32(float(F32(xb / f0)) - math.sin(w) / (2.0 * float(PI_F)))
+        return F32(1.0)
+    if name == "Step":
+        return F32(1.0) if x > 0.0 else F32(0.0)
+    raise ConfError(f"Wrong type({name}) for source time function")
+
+
+def stf_samples(name, start, t0, f0, nt, stept, moment):
+    """The 2(nt+1) half-step samples of one source (source.cpp:72-132):
+    t = i*(stept/2) - start; moment tables flush |s| < 1e-20 to 0."""
+    nstf = 2 * (nt + 1)
+    dt = F32(float(F32(stept)) / 2.0)
+    out = np.empty(nstf, F32)
+    for i in range(nstf):
+        s = _stf_value(name, F32(F32(i) * dt) - F32(start), t0, f0)
+        out[i] = F32(0.0) if (moment and abs(s) < SEIS_ZERO) else s
+    return out
+
+
+def angle2moment_f(strike, dip, rake):
+    """source::angle2moment (source.cpp:134-155) in the float build: the
+    angles are floats, so sin/cos of a float bind sinf/cosf and the 2x-angle
+    terms are double."""
+    S = F32(float(F32(strike)) / 180.0 * float(PI_F))
+    D = F32(float(F32(dip)) / 180.0 * float(PI_F))
+    R = F32(float(F32(rake)) / 180.0 * float(PI_F))
+    sD, cD, sR, cR, sS, cS = _sinf(D), _cosf(D), _sinf(R), _cosf(R), _sinf(S), _cosf(S)
+    s2S, c2S = math.sin(2.0 * float(S)), math.cos(2.0 * float(S))
+    s2D, c2D = math.sin(2.0 * float(D)), math.cos(2.0 * float(D))
+    m11 = F32(-(float(F32(sD * cR)) * s2S + s2D * float(sR) * float(sS) * float(sS)))
+    m22 = F32(float(F32(sD * cR)) * s2S - s2D * float(sR) * float(cS) * float(cS))
+    m33 = F32(-F32(m11 + m22))
+    m12 = F32(float(F32(sD * cR)) * c2S + s2D * float(sR) * s2S * 0.5)
+    m13 = F32(-(float(F32(F32(cD * cR) * cS)) + c2D * float(sR) * float(sS)))
+    m23 = F32(-(float(F32(F32(cD * cR) * sS)) - c2D * float(sR) * float(cS)))
+    return np.array([m11, m22, m33, m12, m13, m23], F32)
+
+
+# ---------------------------------------------------------------------------
+# point location (common.cpp:225-268)
+# ---------------------------------------------------------------------------
+def reposition(px, py, pz, x, y, z, hyp):
+    """common::reposition: the grid point nearest to (px, py, pz) in float
+    squared distance, first in (i, j, k) order among equals.  Points above
+    `hyp` are placed on the surface row by their horizontal distance, and
+    get that point's z.  Returns ghost-inclusive (ix, iy, iz) and the new z."""
+    px, py, pz = F32(px), F32(py), F32(pz)
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj))
+    if pz > hyp:
+        kz = LFD + nk - 1
+        dx = x[v + (kz,)] - px
+        dy = y[v + (kz,)] - py
+        p = dx * dx + dy * dy
+        n = int(np.argmin(p))
+        if not p.flat[n] < SEIS_INF:
+            return 0, 0, kz, z[0, 0, kz]
+        ix, iy = divmod(n, nj)
+        return ix + LFD, iy + LFD, kz, z[ix + LFD, iy + LFD, kz]
+    best, loc = SEIS_INF, (0, 0, 0)
+    vz = slice(LFD, LFD + nk)
+    for i in range(ni):  # one x plane at a time keeps the temporaries small
+        dx = x[LFD + i, v[1], vz] - px
+        dy = y[LFD + i, v[1], vz] - py
+        dz = z[LFD + i, v[1], vz] - pz
+        p = (dx * dx + dy * dy) + dz * dz
+        n = int(np.argmin(p))
+        if p.flat[n] < best:
+            best = p.flat[n]
+            j, k = divmod(n, nk)
+            loc = (i + LFD, j + LFD, k + LFD)
+    return loc[0], loc[1], loc[2], pz
+
+
+def _inside(loc, ni, nj, nk):
+    ix, iy, iz = loc
+    return LFD <= ix < LFD + ni and LFD <= iy < LFD + nj and LFD <= iz < LFD + nk
+
+
+# ---------------------------------------------------------------------------
+# sources (source.cpp:12-70, 671-849, 985-1289)
+# ---------------------------------------------------------------------------
+def _read_netcdf_classic(path, what):
+    from scipy.io import netcdf_file
+
+    try:
+        with open(path, "rb") as f:
+            magic = f.read(4)
+    except OSError as e:
+        raise ConfError(f"fail to open {what} {path}: {e.strerror}") from None
+    if magic[:3] != b"CDF":
+        raise ConfError(f"{what} {path} is not a netCDF classic file (NETCDF4/HDF5 files cannot be read here)")
+    return netcdf_file(path, "r", mmap=False)
+
+
+def read_sources(path, seispath, nt, stept):
+    """Sources of the source file with positions in metres (before
+    location).  Returns a dict with force / moment / focal lists."""
+    c = ConfFile(path, "source configure")
+    stype = c.int("source_type")
+    out = {"force": None, "moment": None, "focal": None}
+    dst2m = c.real("distance2meter")
+    out["hyp"] = F32(c.real("source_hyper_height") * dst2m)
+    fd0 = c.real("simulate_damp")  # read for every source type (source.cpp:786-788)
+    if stype == 1:
+        fn = f"{seispath}/{c.str('focal_process_NC_file')}"
+        with _read_netcdf_classic(fn, "focal process file") as f:
+            npnt, fnt = f.dimensions["NP"], f.dimensions["NT"]
+            fdt = F32(f.DT if np.ndim(f.DT) == 0 else f.DT[0])
+            ser = np.stack([np.asarray(f.variables[nm][:], F32).reshape(npnt, fnt) * fd0
+                            for nm in ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz")])
+            pos = np.stack([np.asarray(f.variables[nm][:], F32).reshape(npnt) for nm in ("posx", "posy", "posz")], 1)
+        out["focal"] = {"pos": pos, "series": ser.astype(F32), "dt": fdt, "nt": fnt}
+        return out
+    if stype not in (2, 3):
+        return out  # any other type leaves the run without sources (source.cpp:39-66)
+    nfrc, nmnt = c.int("number_of_force_source"), c.int("number_of_moment_source")
+    if nfrc:
+        nwin = c.int("force_stf_window")
+        s = c.chunk("<anchor_force>")
+        rows = [s.reals(8) for _ in range(nfrc)]
+        pos = np.array([[F32(r[0] * dst2m), F32(r[1] * dst2m), F32(r[2] * dst2m)] for r in rows], F32)
+        vec = np.array([[F32(r[5] * r[4]), F32(r[6] * r[4]), F32(r[7] * r[4])] for r in rows], F32)
+        stf = np.zeros((nfrc, 2 * (nt + 1)), F32)
+        for i, r in enumerate(rows):
+            for k in range(nwin):  # the last window is what the reference keeps
+                stf[i] = stf_samples(c.str("force_stf_type", 3 + k), r[3], c.real("force_stf_timefactor", 3 + k),
+                                     c.real("force_stf_freqfactor", 3 + k), nt, stept, moment=False)
+        out["force"] = {"pos": pos, "vec": vec, "stf": stf}
+    if nmnt:
+        nwin = c.int("moment_stf_window")
+        mech = c.str("moment_mech_input")
+        if mech not in ("moment", "angle"):
+            raise ConfError(f"moment mechanics input wrong = {mech}")
+        s = c.chunk("<anchor_moment>")
+        pos, vec, start = [], [], []
+        for _ in range(nmnt):
+            r = s.reals(11 if mech == "moment" else 8)
+            m = r[5:11] if mech == "moment" else angle2moment_f(r[5], r[6], r[7])
+            pos.append([F32(r[0] * dst2m), F32(r[1] * dst2m), F32(r[2] * dst2m)])
+            vec.append([F32(v * r[4]) for v in m])
+            start.append(r[3])
+        stf = np.zeros((nmnt, 2 * (nt + 1)), F32)
+        for i in range(nmnt):
+            for k in range(nwin):
+                stf[i] = stf_samples(c.str("moment_stf_type", 3 + k), start[i], c.real("moment_stf_timefactor", 3 + k),
+                                     c.real("moment_stf_freqfactor", 3 + k), nt, stept, moment=True)
+        out["moment"] = {"pos": np.array(pos, F32), "vec": np.array(vec, F32), "stf": stf, "angle": mech == "angle"}
+    return out
+
+
+def locate_sources(src, x, y, z, rho, beta):
+    """source::cal_index (source.cpp:985-1289) without SrcSmooth: each source
+    goes to its nearest grid point; angle-mode moments and focal series are
+    scaled by mu = rho*beta*beta there.  Force and moment sources outside
+    the valid grid are errors; focal points outside it are dropped (the
+    reference zeroes their series)."""
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    hyp = src["hyp"]
+    res = {}
+    for kind in ("force", "moment"):
+        s = src[kind]
+        if s is None:
+            continue
+        locs = []
+        for i, p in enumerate(s["pos"]):
+            ix, iy, iz, pz = reposition(p[0], p[1], p[2], x, y, z, hyp)
+            if not _inside((ix, iy, iz), ni, nj, nk):
+                raise ConfError(f"The {i + 1} {kind} source at position({p[0]},{p[1]},{p[2]}), exceed the coord boundary")
+            s["pos"][i, 2] = pz
+            locs.append((ix, iy, iz))
+        locs = np.array(locs, np.int32)
+        vec = s["vec"].copy()
+        if kind == "moment" and s["angle"]:
+            for i, (ix, iy, iz) in enumerate(locs):
+                vec[i] *= F32(F32(rho[ix, iy, iz] * beta[ix, iy, iz]) * beta[ix, iy, iz])
+        res[kind + "_locs"], res[kind + "_vec"], res[kind + "_stf"] = locs, vec, s["stf"]
+    f = src["focal"]
+    if f is not None:
+        keep, locs = [], []
+        for i, p in enumerate(f["pos"]):
+            loc = reposition(p[0], p[1], p[2], x, y, z, hyp)[:3]
+            if _inside(loc, ni, nj, nk):
+                keep.append(i)
+                locs.append(loc)
+        ser = f["series"][:, keep, :].copy()
+        for n, (ix, iy, iz) in enumerate(locs):
+            ser[:, n, :] = F32(F32(rho[ix, iy, iz] * beta[ix, iy, iz]) * beta[ix, iy, iz]) * ser[:, n, :]
+        res["focal_locs"] = np.array(locs, np.int32).reshape(-1, 3)
+        res["focal_series"] = ser
+        res["focal_dt"] = float(f["dt"])
+    return res
+
+
+# ---------------------------------------------------------------------------
+# stations (seisplot.cpp:12-55, 330-457)
+# ---------------------------------------------------------------------------
+def read_stations(path):
+    """Snapshot specs, receiver/line points (metres) with their Pnum/Lnum,
+    topo_hyper_height and tinv_of_seismo of a station file."""
+    c = ConfFile(path, "seisplot configure")
+    nsnap, nline, nrecv = c.int("number_of_snap"), c.int("number_of_inline"), c.int("number_of_recv")
+    hyp = c.real("topo_hyper_height")
+    tinv = c.int("tinv_of_seismo")
+    snaps = np.array([[c.int(f"snap_{i + 1:03d}", col) for col in range(3, 14)] for i in range(nsnap)],
+                     np.int32).reshape(-1, 11)
+    pos, pnum, lnum = [], [], []
+    for i in range(nrecv):
+        key = f"recv_{i + 1:03d}"
+        pos.append([c.real(key, 3), c.real(key, 4), c.real(key, 5)])
+        pnum.append(i)
+        lnum.append(0)
+    for i in range(nline):
+        key = f"line_{i + 1:03d}"
+        x0, y0, z0 = c.real(key, 3), c.real(key, 4), c.real(key, 5)
+        dx, dy, dz = c.real(key, 7), c.real(key, 8), c.real(key, 9)
+        num = c.real(key, 11)
+        j = 0
+        while j < num:
+            pos.append([F32(x0 + F32(F32(j) * dx)), F32(y0 + F32(F32(j) * dy)), F32(z0 + F32(F32(j) * dz))])
+            pnum.append(j)
+            lnum.append(i + 1)
+            j += 1
+    return {"snapshots": snaps, "pos": np.array(pos, F32).reshape(-1, 3), "pnum": np.array(pnum, np.int32),
+            "lnum": np.array(lnum, np.int32), "hyp": hyp, "tinv": tinv}
+
+
+def locate_stations(st, x, y, z):
+    """seisplot::locpoint (seisplot.cpp:412-440): every receiver and line
+    point goes to its nearest valid grid point; surface points take that
+    point's z."""
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    locs = []
+    for i, p in enumerate(st["pos"]):
+        ix, iy, iz, pz = reposition(p[0], p[1], p[2], x, y, z, st["hyp"])
+        if not _inside((ix, iy, iz), ni, nj, nk):
+            raise ConfError(f"No.{i} reciver is out of Computing Boundary (No Virtual)!")
+        st["pos"][i, 2] = pz
+        locs.append((ix, iy, iz))
+    return np.array(locs, np.int32).reshape(-1, 3)
+
+
+# ---------------------------------------------------------------------------
+# volume media (mediaparas.cpp:294-441 of mathfunc/mediapar, 1273-1425, 2140-2219)
+# ---------------------------------------------------------------------------
+def loc_value_1d(p, arr):
+    """mathfunc::LocValue1d (mathfunc.cpp:294-328), vectorised: left index L
+    of the bracketing pair and the value clamped to the array's ends."""
+    p = np.asarray(p, F32)
+    n = arr.size
+    L = np.searchsorted(arr, p, side="right") - 1
+    L = np.clip(L, 0, n - 2)
+    val = p.copy()
+    lo, hi = p <= arr[0], p >= arr[n - 1]
+    L[lo], val[lo] = 0, arr[0]
+    L[hi & ~lo], val[hi & ~lo] = n - 2, arr[n - 1]
+    return L, val
+
+
+def _lagrange2(X0, X1, x0):
+    """Weights of interp2d/interp3d with two nodes (mathfunc.cpp:353-447):
+    L0 = (X1-x0)/(X1-X0), L1 = (X0-x0)/(X0-X1), float."""
+    return ((X1 - x0) / (X1 - X0)).astype(F32), ((X0 - x0) / (X0 - X1)).astype(F32)
+
+
+def volume_media(path, x, y, z, zwmedia=True):
+    """Media of a volume file (netCDF classic: coordinate variables x, y and
+    z -- or depth when the reference is built with ZWmedia, as its Makefile
+    does -- and Vp, Vs, rho on [x][y][z]) mapped onto the grid: trilinear
+    Lagrange interpolation at each cell (volume_discrete), get2in1, then
+    MediaStatistics and the ghost extension of readdata.  Returns
+    (alpha, beta, rho) with ghosts."""
+    zname = "depth" if zwmedia else "z"
+    with _read_netcdf_classic(path, "media volume file") as f:
+        X = np.asarray(f.variables["x"][:], F32)
+        Y = np.asarray(f.variables["y"][:], F32)
+        Z = np.asarray(f.variables[zname][:], F32)
+        vp, vs, den = (np.asarray(f.variables[nm][:], F32).reshape(X.size, Y.size, Z.size) for nm in ("Vp", "Vs", "rho"))
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj), slice(LFD, LFD + nk))
+    # the reference first cuts the volume to the grid's coordinate range
+    # (read_volume_new); the bracketing cells and weights are the same
+    cut = []
+    for arr, c in ((X, x), (Y, y), (Z, z)):
+        lo, _ = loc_value_1d(np.array([c[v].min()], F32), arr)
+        hi, _ = loc_value_1d(np.array([c[v].max()], F32), arr)
+        cut.append(slice(int(lo[0]), int(hi[0]) + 2))
+    X, Y, Z = X[cut[0]], Y[cut[1]], Z[cut[2]]
+    vp, vs, den = (a[cut[0], cut[1], cut[2]] for a in (vp, vs, den))
+    i1, xv = loc_value_1d(x[v], X)
+    j1, yv = loc_value_1d(y[v], Y)
+    k1, zv = loc_value_1d(z[v], Z)
+    lx = _lagrange2(X[i1], X[i1 + 1], xv)
+    ly = _lagrange2(Y[j1], Y[j1 + 1], yv)
+    lz = _lagrange2(Z[k1], Z[k1 + 1], zv)
+
+    def interp(F):
+        acc = np.zeros(xv.shape, F32)
+        for a in range(2):
+            for b in range(2):
+                for c in range(2):
+                    w = (lx[a] * ly[b]) * lz[c]
+                    acc = acc + w * F[i1 + a, j1 + b, k1 + c]
+        return acc
+
+    Vp, Vs, Dc = interp(vp), interp(vs), interp(den)
+    # get2in1 with equal upper/lower values (mediaparas.cpp:872-886)
+    mu = (Dc * Vs) * Vs
+    lam = ((Vp * Vp) * Dc).astype(np.float64) - 2.0 * mu.astype(np.float64)
+    lam = lam.astype(F32)
+    density = ((Dc.astype(np.float64) + Dc) / 2.0).astype(F32)
+    miu = ((mu.astype(np.float64) + mu) / 2.0).astype(F32)
+    lamb = ((lam.astype(np.float64) + lam) / 2.0).astype(F32)
+    # MediaStatistics (mediaparas.cpp:1837-1905)
+    shape = x.shape
+    alpha, beta, rho = np.zeros(shape, F32), np.zeros(shape, F32), np.zeros(shape, F32)
+    beta[v] = np.sqrt(miu / density)
+    alpha[v] = np.sqrt((lamb.astype(np.float64) + 2.0 * miu.astype(np.float64)) / density).astype(F32)
+    rho[v] = density
+    for a in (alpha, beta, rho):
+        _sc.equivalence_extend(a)
+    return alpha, beta, rho
+
+
+# ---------------------------------------------------------------------------
+# the whole run
+# ---------------------------------------------------------------------------
+def load(conf_path, seispath=None, media=None, zwmedia=True):
+    """Scenario dict for `cgfd3d.Simulation.setup` from a SeisFD3D.conf.
+
+    `seispath` overrides the directory the conf file names; `media` =
+    (alpha, beta, rho) arrays with ghosts replace the configured media.
+    The dict also holds the receiver metadata (`recv_pos`, `recv_pnum`,
+    `recv_lnum`, `seismo_tinv`) that `Simulation.write_seismo` takes."""
+    m = read_main(conf_path, seispath)
+    ni, nj, nk, nt = m["ni"], m["nj"], m["nk"], m["nt"]
+    if m["grid_type"] == "point":
+        raise ConfError("grid_type point: the reference's read_point stores x, y and z all into crd.x "
+                        "(gridmesh.cpp:52-68), so it yields no valid grid; use a vmap grid")
+    if m["grid_type"] != "vmap":
+        raise ConfError(f"gridtype configure {m['grid_type']} is wrong when reading grid data")
+    if m["hybrid_grid"] != 0:
+        raise ConfError("hybrid_grid 1 needs the hybrid Cartesian kernels, which this engine does not have; "
+                        "set hybrid_grid = 0 for the all-curvilinear method")
+    posx, posy, posz, gp, eq = read_vmap(m["grid_filename"], ni, nj, nk)
+    x, y, z = vmap_coords(posx, posy, posz, gp, eq, nk)
+    met = _sc.metrics_from_coords(x, y, z, m["steph"])
+
+    if media is not None:
+        alpha, beta, rho = (np.asarray(a, F32) for a in media)
+    elif m["media_type"] == "volume":
+        alpha, beta, rho = volume_media(m["media_filename"], x, y, z, zwmedia)
+    elif m["media_type"] == "interface":
+        raise ConfError("media_type interface: the reference's readdata overwrites the interface media with zero "
+                        "Vp/Vs/rho (get2in1, mediaparas.cpp:2163-2200) and MediaStatistics turns them into NaN; "
+                        "use volume media or pass media arrays")
+    else:
+        raise ConfError(f"media_type {m['media_type']} is not supported by this front end; pass media arrays")
+
+    nabs, vel, bmax, amax = read_absorb(m["absorb_filename"])
+    pml = _sc.cfs_pml_tables(ni, nj, nk, nabs, vel, bmax, amax, m["steph"], met)
+
+    src = read_sources(m["source_filename"], m["seispath"], nt, m["stept"])
+    sc = dict(ni=ni, nj=nj, nk=nk, nt=nt, dh=float(m["steph"]), dt=float(m["stept"]), metrics=met,
+              alpha=alpha, beta=beta, rho=rho, nabs=nabs, pml=pml, pgv=bool(m["output_peakvel"]),
+              coords=(x, y, z), name=f"conf-{os.path.basename(os.path.dirname(os.path.abspath(conf_path)))}")
+    sc.update(locate_sources(src, x, y, z, rho, beta))
+    st = read_stations(m["station_filename"])
+    sc["recv_locs"] = locate_stations(st, x, y, z)
+    sc["recv_pos"], sc["recv_pnum"], sc["recv_lnum"], sc["seismo_tinv"] = st["pos"], st["pnum"], st["lnum"], st["tinv"]
+    sc["snapshots"] = st["snapshots"]
+    return sc

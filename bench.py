@@ -15,8 +15,8 @@ metrics are the constants the mesh implies, + 144 B per PML cell-axis, FP32)
 by the mean duration of the fused stage kernel measured with HIP events on
 the engine's stream over the timed steps; `traffic` is the measured HBM bytes
 per launch from the committed counter summary.  The CPU baseline is the oracle
-(a C/OpenMP restatement of the reference kernels) timed on this host on one
-step of the same problem.
+(a C/OpenMP restatement of the reference kernels) timed on this host on three
+steps (about 15 s of CPU work) of the same problem.
 """
 import argparse
 import json
@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat", "batch"])
     ap.add_argument("--batch", type=int, default=16, help="scenarios of the batch workload")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0,

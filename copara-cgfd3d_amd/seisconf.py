@@ -26,6 +26,7 @@ Supported inputs and the reference paths they follow:
 | source_type 1: focal file in netCDF classic format | source.cpp:43-58, 787-848, 1185-1262 |
 | station file: snapshots, receivers, lines | seisplot.cpp:12-55, 330-457 |
 | media_type volume in netCDF classic format | mediaparas.cpp:376-441, 1273-1425, 2140-2219 |
+| media_type interface3D | mediaparas.cpp:34-43, 52-374, 442-914, 1837-1905 |
 
 Refused, with the reason in the error:
 - grid_type point: `read_point` stores x, y and z all into `crd.x`
@@ -36,7 +37,6 @@ Refused, with the reason in the error:
 - hybrid_grid 1: needs the hybrid Cartesian kernels, which are not part of
   this engine.
 - NETCDF4/HDF5 media and focal files: no HDF5 reader is available here.
-- media_type interface3D: not restated; pass media arrays instead.
 
 Callers may always pass their own `media=(alpha, beta, rho)` arrays instead.
 """
@@ -741,6 +741,213 @@ def volume_media(path, x, y, z, zwmedia=True):
 
 
 # ---------------------------------------------------------------------------
+# 3-D interface media (mediaparas.cpp:52-374, 442-836, 838-914, 1837-1905)
+# ---------------------------------------------------------------------------
+def _interp2d_w(X, Y, x0, y0):
+    """Two-node Lagrange weights of interp2d (mathfunc.cpp:353-393), float."""
+    lx = (F32(F32(X[1] - x0) / F32(X[1] - X[0])), F32(F32(X[0] - x0) / F32(X[0] - X[1])))
+    ly = (F32(F32(Y[1] - y0) / F32(Y[1] - Y[0])), F32(F32(Y[0] - y0) / F32(Y[0] - Y[1])))
+    return lx, ly
+
+
+def _interp2d(lx, ly, Z):
+    acc = F32(0.0)
+    for a in range(2):
+        for b in range(2):
+            acc = F32(acc + F32(F32(lx[a] * ly[b]) * Z[a, b]))
+    return acc
+
+
+def read_interface3d(path, nx, ny, x0, y0, dx, dy, nint):
+    """An interface3D media file and the tables read_interface3D derives
+    from it (mediaparas.cpp:52-260): depths Zpos[i][j][n] and Vp/Vs/Dc[m][i][j][n]
+    (m = 0 above, 1 below interface n), flatness, constancy, layer depth
+    ranges and the coincident-interface index table indexZ[n][i][j][2].
+    x0/y0/dx/dy are the integers `get_conf` reads (atoi)."""
+    c = ConfFile(path, "media interface")
+    s = _Scanner([t for p in c.pieces for t in p.split()], path)
+    Z = s.reals(nint * ny * nx).reshape(nint, ny, nx).transpose(2, 1, 0).copy()
+    V = {}
+    for nm in ("Vp", "Vs", "Dc"):
+        V[nm] = s.reals(nint * 2 * ny * nx).reshape(nint, 2, ny, nx).transpose(1, 3, 2, 0).copy()
+    bad = np.argwhere(Z[:, :, :-1] < Z[:, :, 1:])
+    if len(bad):
+        i, j, n = bad[0]
+        raise ConfError(f"at location ({i},{j},{n}) interface depth[{n}] is lower than depth[{n + 1}]")
+    vp, vs = V["Vp"].astype(np.float64), V["Vs"].astype(np.float64)
+    air = np.zeros(vp.shape, bool)
+    air[0, :, :, 0] = (V["Vp"][0, :, :, 0] == 0) & (V["Vs"][0, :, :, 0] == 0)
+    if ((vp * vp <= 2.0 * vs * vs) & ~air).any():
+        raise ConfError("there is a velocity error: Vp^2 <= 2 Vs^2")
+    t = {"Z": Z, "V": V, "nint": nint,
+         "X": np.array([F32(x0 + i * dx) for i in range(nx)], F32),
+         "Y": np.array([F32(y0 + j * dy) for j in range(ny)], F32)}
+    t["flat"] = [bool((np.abs(Z[0, 0, n] - Z[:, :, n]) <= SEIS_ZERO).all()) for n in range(nint)]
+    const = np.zeros((nint, 2), bool)
+    for n in range(nint):
+        for m in range(2):
+            const[n, m] = all((np.abs(V[nm][m, 0, 0, n] - V[nm][m, :, :, n]) <= SEIS_ZERO).all()
+                              for nm in ("Vp", "Vs", "Dc"))
+    t["const"] = const
+    lconst = []
+    for n in range(nint):
+        if n == nint - 1 and const[n, 1]:
+            lconst.append(True)
+        elif const[n, 1] and n + 1 < nint and const[n + 1, 0] and all(
+                abs(V[nm][1, 0, 0, n] - V[nm][0, 0, 0, n + 1]) < SEIS_ZERO for nm in ("Vp", "Vs", "Dc")):
+            lconst.append(True)
+        else:
+            lconst.append(False)
+    t["layer_const"] = lconst
+    rng = np.zeros((nint, 2), F32)
+    for n in range(nint - 1):
+        rng[n] = (max(Z[0, 0, n], Z[:, :, n].max()), min(Z[0, 0, n + 1], Z[:, :, n + 1].min()))
+    n = nint - 1
+    rng[n] = (Z[:, :, n].max(), F32(float(Z[:, :, n].min()) - 6371e3))
+    t["layer_range"] = rng
+    idx = np.zeros((nint, nx, ny, 2), np.int64)
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            corners = Z[i:i + 2, j:j + 2, :]
+            idx[nint - 1, i, j, 1] = nint
+            for n in range(nint - 2, -1, -1):
+                h = np.abs(corners[:, :, n] - corners[:, :, n + 1]).max()
+                idx[n, i, j, 1] = idx[n + 1, i, j, 1] if h < SEIS_ZERO else n + 1
+            top_air = const[0, 0] and V["Vp"][0, 0, 0, 0] == 0
+            idx[0, i, j, 0] = -idx[0, i, j, 1] if top_air else 1
+            for n in range(1, nint):
+                h = np.abs(corners[:, :, n] - corners[:, :, n - 1]).max()
+                idx[n, i, j, 0] = idx[n - 1, i, j, 0] if h < SEIS_ZERO else n + 1
+    idx[:, :, ny - 1, :] = idx[:, :, ny - 2, :]
+    idx[:, nx - 1, :, :] = idx[:, nx - 2, :, :]
+    t["indexZ"] = idx
+    return t
+
+
+def _i3d_column(t, x0, y0):
+    """Everything interface3d_discrete needs at horizontal position (x0, y0):
+    the cell (i1, j1), the interface depths there and each interface side's
+    (Vp, Vs, Dc)."""
+    i1, xv = loc_value_1d(np.array([x0], F32), t["X"])
+    j1, yv = loc_value_1d(np.array([y0], F32), t["Y"])
+    i1, j1, xv, yv = int(i1[0]), int(j1[0]), xv[0], yv[0]
+    lx, ly = _interp2d_w(t["X"][i1:i1 + 2], t["Y"][j1:j1 + 2], xv, yv)
+    nint = t["nint"]
+    zs = [t["Z"][i1, j1, n] if t["flat"][n] else _interp2d(lx, ly, t["Z"][i1:i1 + 2, j1:j1 + 2, n])
+          for n in range(nint)]
+    side = {}
+    for n in range(nint):
+        for m in range(2):
+            if t["const"][n, m]:
+                side[n, m] = tuple(t["V"][nm][m, i1, j1, n] for nm in ("Vp", "Vs", "Dc"))
+            else:
+                side[n, m] = tuple(_interp2d(lx, ly, t["V"][nm][m, i1:i1 + 2, j1:j1 + 2, n])
+                                   for nm in ("Vp", "Vs", "Dc"))
+    inner = {}
+    for n in range(nint):  # values interpolated regardless of constancy (layer interior)
+        for m in range(2):
+            inner[n, m] = tuple(_interp2d(lx, ly, t["V"][nm][m, i1:i1 + 2, j1:j1 + 2, n]) for nm in ("Vp", "Vs", "Dc"))
+    return {"i1": i1, "j1": j1, "z": zs, "side": side, "inner": inner}
+
+
+def _i3d_point(t, col, z0, prev):
+    """interface3d_discrete (mediaparas.cpp:442-836) at depth z0 of a column:
+    the (upper, lower) (Vp, Vs, Dc) pair; `prev` is returned when no layer
+    matches, as the reference keeps the previous point's values."""
+    nint, idx, i1, j1 = t["nint"], t["indexZ"], col["i1"], col["j1"]
+
+    def upper(n):
+        k, s = idx[n, i1, j1, 0], 0
+        if k < 0:
+            k, s = -k, 1
+        return col["side"][k - 1, s]
+
+    def lower(n):
+        return col["side"][idx[n, i1, j1, 1] - 1, 1]
+
+    n1, n2 = nint + 1, -1
+    for n in range(nint):
+        if t["layer_range"][n, 1] <= z0 <= t["layer_range"][n, 0]:
+            n1, n2 = min(n1, n), max(n2, n + 1)
+    n2 = min(n2, nint - 1)
+    if n1 > n2:
+        v = upper(0)
+        return v, v
+    for n in range(n1, n2 + 1):
+        if idx[n, i1, j1, 1] != n + 1:
+            continue
+        z1 = col["z"][n]
+        if abs(F32(z0 - z1)) < SEIS_ZERO:
+            return upper(n), lower(n)
+        if z0 > z1:
+            v = upper(n)
+            return v, v
+        if n == nint - 1:
+            v = lower(n)
+            return v, v
+        z2 = col["z"][n + 1]
+        if abs(F32(z0 - z2)) < SEIS_ZERO:
+            return upper(n + 1), lower(n + 1)
+        if z0 > z2:
+            if t["layer_const"][n]:
+                v = tuple(t["V"][nm][1, i1, j1, n] for nm in ("Vp", "Vs", "Dc"))
+                return v, v
+            L = F32(z1 - z2)
+            L1 = F32(F32(z0 - z2) / L)
+            L2 = F32(F32(1.0) - L1)
+            a, b = col["inner"][n, 1], col["inner"][n + 1, 0]
+            v = tuple(F32(F32(a[q] * L1) + F32(b[q] * L2)) for q in range(3))
+            return v, v
+    return prev
+
+
+def _two_in_one(v0, v1):
+    """TwoinOne + average for one sample (mediaparas.cpp:838-914): density is
+    the mean, lambda and mu the harmonic means of the two sides (mu = 0 when
+    either side is fluid)."""
+    (vp0, vs0, dc0), (vp1, vs1, dc1) = v0, v1
+    miu = [F32(F32(dc * vs) * vs) for dc, vs in ((dc0, vs0), (dc1, vs1))]
+    lam = [F32(float(F32(F32(vp * vp) * dc)) - 2.0 * float(m)) for vp, dc, m in ((vp0, dc0, miu[0]), (vp1, dc1, miu[1]))]
+    rhoval = F32(0.0 + 0.5 * float(F32(dc0 + dc1)))
+    if miu[0] < SEIS_ZERO or miu[1] < SEIS_ZERO:
+        muval = F32(0.0)
+    else:
+        muval = F32(0.0 + 0.5 / float(miu[0]) + 0.5 / float(miu[1]))
+    lamval = F32(0.0 + 0.5 / float(lam[0]) + 0.5 / float(lam[1]))
+    density = rhoval
+    mu = F32(1.0 / float(muval)) if muval > 0 else F32(0.0)
+    lamb = F32(1.0 / float(lamval))
+    return density, mu, lamb
+
+
+def interface3d_media(path, nx, ny, x0, y0, dx, dy, nint, x, y, z):
+    """Media of an interface3D file on the grid: interface3d_discrete at each
+    valid cell, TwoinOne/average, MediaStatistics and the ghost extension.
+    Returns (alpha, beta, rho) with ghosts.  Pure Python per cell: meant for
+    grids up to a few million cells."""
+    t = read_interface3d(path, nx, ny, x0, y0, dx, dy, nint)
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    shape = x.shape
+    alpha, beta, rho = np.zeros(shape, F32), np.zeros(shape, F32), np.zeros(shape, F32)
+    prev = ((F32(0.0),) * 3, (F32(0.0),) * 3)
+    for i in range(LFD, LFD + ni):
+        for j in range(LFD, LFD + nj):
+            cols = {}
+            for k in range(LFD, LFD + nk):
+                key = (x[i, j, k], y[i, j, k])
+                if key not in cols:
+                    cols[key] = _i3d_column(t, *key)
+                prev = _i3d_point(t, cols[key], z[i, j, k], prev)
+                dc, mu, lamb = _two_in_one(*prev)
+                beta[i, j, k] = np.sqrt(F32(mu / dc))
+                alpha[i, j, k] = F32(math.sqrt((float(lamb) + 2.0 * float(mu)) / float(dc)))
+                rho[i, j, k] = dc
+    for a in (alpha, beta, rho):
+        _sc.equivalence_extend(a)
+    return alpha, beta, rho
+
+
+# ---------------------------------------------------------------------------
 # the whole run
 # ---------------------------------------------------------------------------
 def load(conf_path, seispath=None, media=None, zwmedia=True):
@@ -768,6 +975,10 @@ def load(conf_path, seispath=None, media=None, zwmedia=True):
         alpha, beta, rho = (np.asarray(a, F32) for a in media)
     elif m["media_type"] == "volume":
         alpha, beta, rho = volume_media(m["media_filename"], x, y, z, zwmedia)
+    elif m["media_type"] == "interface3D":
+        c = ConfFile(conf_path, "main par")
+        par = [c.int("interface3dpar", 3 + q) for q in range(7)]
+        alpha, beta, rho = interface3d_media(m["media_filename"], *par, x, y, z)
     elif m["media_type"] == "interface":
         raise ConfError("media_type interface: the reference's readdata overwrites the interface media with zero "
                         "Vp/Vs/rho (get2in1, mediaparas.cpp:2163-2200) and MediaStatistics turns them into NaN; "

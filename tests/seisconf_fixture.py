@@ -147,3 +147,42 @@ def write_focal(path, nfnt=40, fdt=0.005):
         for c, nm in enumerate(("posx", "posy", "posz")):
             v = f.createVariable(nm, "f4", ("NP",))
             v[:] = pos[:, c]
+
+
+I3D = dict(nx=7, ny=6, x0=-500, y0=-500, dx=500, dy=500)
+ROCK_A_TOP = (4000.0, 2300.0, 2500.0)    # lower side of interface 0
+ROCK_A_BOT = (4400.0, 2500.0, 2550.0)    # upper side of interface 1: layer 0 is a gradient
+ROCK_B = (5000.0, 2900.0, 2700.0)        # layer 1, constant
+ROCK_C = (6000.0, 3400.0, 2900.0)        # below interface 2
+
+
+def i3d_plane(x, y):
+    """Interface 1: a tilted plane (bilinear interpolation reproduces it)."""
+    return -600.0 + 0.05 * x - 0.08 * y
+
+
+def write_interface3d(d):
+    """An interface3D media file (three interfaces: flat at +400 m with air
+    above, a tilted plane, flat at -900 m) and a conf that points to it."""
+    p = I3D
+    X = p["x0"] + np.arange(p["nx"]) * p["dx"]
+    Y = p["y0"] + np.arange(p["ny"]) * p["dy"]
+    XX, YY = np.meshgrid(X, Y, indexing="ij")
+    zs = [np.full_like(XX, 400.0, dtype=float), i3d_plane(XX, YY), np.full_like(XX, -900.0, dtype=float)]
+    sides = [((0.0, 0.0, 0.0), ROCK_A_TOP), (ROCK_A_BOT, ROCK_B), (ROCK_B, ROCK_C)]
+    out = []
+    for n in range(3):
+        for j in range(p["ny"]):
+            out.append(" ".join(f"{zs[n][i, j]:.6f}" for i in range(p["nx"])))
+    for q in range(3):  # Vp, Vs, Dc
+        for n in range(3):
+            for m in range(2):
+                for j in range(p["ny"]):
+                    out.append(" ".join(f"{sides[n][m][q]:.1f}" for _ in range(p["nx"])))
+    (d / "media.int3d").write_text("\n".join(out) + "\n")
+    conf = d / "SeisFD3D.conf"
+    txt = conf.read_text().replace("media_type = volume", "media_type = interface3D").replace(
+        "media_filename = vol.nc", "media_filename = media.int3d")
+    txt += f"interface3dpar = {p['nx']} {p['ny']} {p['x0']} {p['y0']} {p['dx']} {p['dy']} 3\n"
+    conf.write_text(txt)
+    return conf

@@ -340,3 +340,59 @@ def test_reference_example_files():
         S.load(f"{EXAMPLE}/SeisFD3D.conf", seispath=EXAMPLE)
     with pytest.raises(S.ConfError, match="classic"):   # its focal file is NETCDF4
         S.read_sources(m["source_filename"], EXAMPLE, 10, m["stept"])
+
+
+def _moduli(v):
+    vp, vs, dc = (np.float64(a) for a in v)
+    mu = dc * vs * vs
+    return mu, dc * vp * vp - 2 * mu, dc
+
+
+def test_interface3d_media_layers(tmp_path):
+    FX.write_set(tmp_path, media=False)
+    conf = FX.write_interface3d(tmp_path)
+    sc = S.load(str(conf))
+    x, y, z = (c.astype(np.float64) for c in sc["coords"])
+    v = (slice(3, -3),) * 3
+    X, Y, Z = x[v], y[v], z[v]
+    alpha, beta, rho = (sc[k][v].astype(np.float64) for k in ("alpha", "beta", "rho"))
+    zp = FX.i3d_plane(X, Y)
+    # layer 0: linear in z between the sides of interfaces 0 (z = 400) and 1 (the plane)
+    a = Z > zp + 1e-3
+    L1 = (Z - zp) / (400.0 - zp)
+    for q, got in ((0, alpha), (1, beta), (2, rho)):
+        want = FX.ROCK_A_TOP[q] * L1 + FX.ROCK_A_BOT[q] * (1 - L1)
+        np.testing.assert_allclose(got[a], want[a], rtol=3e-5)
+    # layer 1 and below interface 2
+    b = (Z < zp - 1e-3) & (Z > -900.0)
+    c = Z < -900.0
+    for mask, rock in ((b, FX.ROCK_B), (c, FX.ROCK_C)):
+        for q, got in ((0, alpha), (1, beta), (2, rho)):
+            np.testing.assert_allclose(got[mask], rock[q], rtol=2e-6)
+    # cells on interface 2: mean density, harmonic-mean moduli of the two sides
+    on = Z == -900.0
+    assert on.sum() == FX.NI * FX.NJ
+    mu0, la0, d0 = _moduli(FX.ROCK_B)
+    mu1, la1, d1 = _moduli(FX.ROCK_C)
+    mu = 1 / (0.5 / mu0 + 0.5 / mu1)
+    la = 1 / (0.5 / la0 + 0.5 / la1)
+    dc = 0.5 * (d0 + d1)
+    np.testing.assert_allclose(rho[on], dc, rtol=1e-6)
+    np.testing.assert_allclose(beta[on], np.sqrt(mu / dc), rtol=2e-6)
+    np.testing.assert_allclose(alpha[on], np.sqrt((la + 2 * mu) / dc), rtol=2e-6)
+    assert a.sum() and b.sum() and c.sum()
+
+
+def test_interface3d_reads_integer_geometry(tmp_path):
+    # interface3dpar's origin and spacing are int fields read with atoi:
+    # "-5.0e2" is -5, as in the reference
+    FX.write_set(tmp_path, media=False)
+    conf = FX.write_interface3d(tmp_path)
+    t = S.read_interface3d(str(tmp_path / "media.int3d"), 7, 6, S.ConfFile(str(conf)).int("interface3dpar", 5),
+                           -500, 500, 500, 3)
+    assert t["X"][0] == -500 and t["X"][1] == 0
+    conf.write_text(conf.read_text().replace("interface3dpar = 7 6 -500", "interface3dpar = 7 6 -5.0e2"))
+    assert S.ConfFile(str(conf)).int("interface3dpar", 5) == -5
+    assert t["flat"] == [True, False, True] and t["layer_const"] == [False, True, True]
+    assert (t["indexZ"][:, :, :, 1] == np.arange(1, 4)[:, None, None]).all()
+    assert (t["indexZ"][0, :, :, 0] == -1).all()       # air above the top interface

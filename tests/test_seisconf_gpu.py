@@ -59,3 +59,15 @@ def test_conf_focal_run_matches_oracle(tmp_path):
     for c in range(9):
         assert_same(g.get_field(c), o.get_field(c), f"field {c}")
     assert np.abs(g.get_seismo()).max() > 0
+
+
+def test_conf_interface3d_media_run(tmp_path):
+    """Layered interface3D media (gradient layer over a tilted interface,
+    harmonic averaging on a grid row) from the parameter files."""
+    nt = 8
+    FX.write_set(tmp_path, nt=nt, media=False)
+    sc = seisconf.load(str(FX.write_interface3d(tmp_path)))
+    g, o = _run(sc, nt)
+    assert_same(g.get_seismo(), o.get_seismo(), "seismograms")
+    for c in range(9):
+        assert_same(g.get_field(c), o.get_field(c), f"field {c}")

@@ -18,3 +18,7 @@ hipcc -shared -fPIC --offload-arch="$ARCH" -o "$here/libcgfd3d.so.tmp" "$here/_b
       "$here/_build/cgfd_ncio.o"
 mv -f "$here/libcgfd3d.so.tmp" "$here/libcgfd3d.so"
 echo "built $here/libcgfd3d.so"
+# the C++ host driver (one child-rank run through the C-ABI)
+g++ -O2 -std=c++17 -Wall -I"$root/include" "$here/host/cgfd_child.cpp" -L"$here" -lcgfd3d \
+    -Wl,-rpath,'$ORIGIN' -o "$here/cgfd_child"
+echo "built $here/cgfd_child"

@@ -27,6 +27,7 @@ Supported inputs and the reference paths they follow:
 | station file: snapshots, receivers, lines | seisplot.cpp:12-55, 330-457 |
 | media_type volume in netCDF classic format | mediaparas.cpp:376-441, 1273-1425, 2140-2219 |
 | media_type interface3D | mediaparas.cpp:34-43, 52-374, 442-914, 1837-1905 |
+| sampling_point_per_cell > 1 (media smoothing) | mediaparas.cpp:1712-1835 |
 
 Refused, with the reason in the error:
 - grid_type point: `read_point` stores x, y and z all into `crd.x`
@@ -188,15 +189,15 @@ def read_main(conf_path, seispath=None):
         "grid_type": c.str("grid_type"), "hybrid_grid": c.int("hybrid_grid"),
         "media_type": c.str("media_type"),
         "sampling": [c.int("sampling_point_per_cell", 3 + i) for i in range(3)],
+        "crit_value": c.real("threshold_velocity_jump_of_interface"),
+        "crit_perc": c.real("threshold_percent_change_of_interface"),
         "output_peakvel": c.int("output_peakvel"),
     }
     for key in ("grid_filename", "media_filename", "source_filename", "station_filename", "absorb_filename"):
         m[key] = f"{m['seispath']}/{c.str(key)}"  # sprintf("%s/%s", parpath, name)
     if m["ni"] <= 0 or m["nj"] <= 0 or m["nk"] <= 0 or m["nt"] <= 0:
         raise ConfError(f"{conf_path}: ni, nj, nk and nt must be positive")
-    if max(m["sampling"]) > 1:
-        raise ConfError("sampling_point_per_cell above 1 asks for the media smoothing of ApplySmooth "
-                        "(mediaparas.cpp:1712-1824), which this front end does not restate")
+
     return m
 
 
@@ -681,13 +682,11 @@ def _lagrange2(X0, X1, x0):
     return ((X1 - x0) / (X1 - X0)).astype(F32), ((X0 - x0) / (X0 - X1)).astype(F32)
 
 
-def volume_media(path, x, y, z, zwmedia=True):
-    """Media of a volume file (netCDF classic: coordinate variables x, y and
-    z -- or depth when the reference is built with ZWmedia, as its Makefile
-    does -- and Vp, Vs, rho on [x][y][z]) mapped onto the grid: trilinear
-    Lagrange interpolation at each cell (volume_discrete), get2in1, then
-    MediaStatistics and the ghost extension of readdata.  Returns
-    (alpha, beta, rho) with ghosts."""
+def _volume_table(path, x, y, z, zwmedia):
+    """read_volume_new (mediaparas.cpp:1273-1425): the volume (netCDF
+    classic: coordinate variables x, y and z -- or depth when the reference
+    is built with ZWmedia, as its Makefile does -- and Vp, Vs, rho on
+    [x][y][z]) cut to the grid's coordinate range."""
     zname = "depth" if zwmedia else "z"
     with _read_netcdf_classic(path, "media volume file") as f:
         X = np.asarray(f.variables["x"][:], F32)
@@ -696,48 +695,174 @@ def volume_media(path, x, y, z, zwmedia=True):
         vp, vs, den = (np.asarray(f.variables[nm][:], F32).reshape(X.size, Y.size, Z.size) for nm in ("Vp", "Vs", "rho"))
     ni, nj, nk = (s - 2 * LFD for s in x.shape)
     v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj), slice(LFD, LFD + nk))
-    # the reference first cuts the volume to the grid's coordinate range
-    # (read_volume_new); the bracketing cells and weights are the same
     cut = []
     for arr, c in ((X, x), (Y, y), (Z, z)):
         lo, _ = loc_value_1d(np.array([c[v].min()], F32), arr)
         hi, _ = loc_value_1d(np.array([c[v].max()], F32), arr)
         cut.append(slice(int(lo[0]), int(hi[0]) + 2))
-    X, Y, Z = X[cut[0]], Y[cut[1]], Z[cut[2]]
-    vp, vs, den = (a[cut[0], cut[1], cut[2]] for a in (vp, vs, den))
-    i1, xv = loc_value_1d(x[v], X)
-    j1, yv = loc_value_1d(y[v], Y)
-    k1, zv = loc_value_1d(z[v], Z)
+    return {"X": X[cut[0]], "Y": Y[cut[1]], "Z": Z[cut[2]],
+            "F": [a[cut[0], cut[1], cut[2]] for a in (vp, vs, den)]}
+
+
+def _volume_sample(vt, px, py, pz):
+    """volume_discrete (mediaparas.cpp:376-441) at arrays of points:
+    (Vp, Vs, Dc) by two-node Lagrange interpolation in each direction."""
+    X, Y, Z = vt["X"], vt["Y"], vt["Z"]
+    i1, xv = loc_value_1d(px, X)
+    j1, yv = loc_value_1d(py, Y)
+    k1, zv = loc_value_1d(pz, Z)
     lx = _lagrange2(X[i1], X[i1 + 1], xv)
     ly = _lagrange2(Y[j1], Y[j1 + 1], yv)
     lz = _lagrange2(Z[k1], Z[k1 + 1], zv)
-
-    def interp(F):
+    out = []
+    for F in vt["F"]:
         acc = np.zeros(xv.shape, F32)
         for a in range(2):
             for b in range(2):
                 for c in range(2):
                     w = (lx[a] * ly[b]) * lz[c]
                     acc = acc + w * F[i1 + a, j1 + b, k1 + c]
-        return acc
+        out.append(acc)
+    return out
 
-    Vp, Vs, Dc = interp(vp), interp(vs), interp(den)
-    # get2in1 with equal upper/lower values (mediaparas.cpp:872-886)
+
+def _get2in1(Vp, Vs, Dc):
+    """get2in1 (mediaparas.cpp:872-886) with equal upper and lower values."""
     mu = (Dc * Vs) * Vs
-    lam = ((Vp * Vp) * Dc).astype(np.float64) - 2.0 * mu.astype(np.float64)
-    lam = lam.astype(F32)
+    lam = (((Vp * Vp) * Dc).astype(np.float64) - 2.0 * mu.astype(np.float64)).astype(F32)
     density = ((Dc.astype(np.float64) + Dc) / 2.0).astype(F32)
     miu = ((mu.astype(np.float64) + mu) / 2.0).astype(F32)
     lamb = ((lam.astype(np.float64) + lam) / 2.0).astype(F32)
-    # MediaStatistics (mediaparas.cpp:1837-1905)
+    return density, miu, lamb
+
+
+def _vel3d(vp0, vp1, crit_value, crit_perc):
+    """set_vel3d (mediaparas.cpp:916-923): the mean Vp, pushed up by 9000 km
+    where the two sides differ by more than the thresholds."""
+    vp0, vp1 = np.asarray(vp0, F32), np.asarray(vp1, F32)
+    d = np.abs(vp0 - vp1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        smooth = (d < crit_value) & (d / np.minimum(vp0, vp1) < crit_perc)
+    mean = (vp0 + vp1).astype(np.float64) / 2.0
+    return np.where(smooth, mean, mean + 9000e3).astype(F32)
+
+
+def _lag3(x0):
+    """Three-node Lagrange weights on nodes 0, 2, 4 (interp3d with vec3,
+    mathfunc.cpp:395-447), float."""
+    X = (F32(0.0), F32(2.0), F32(4.0))
+    L = []
+    for i in range(3):
+        w = F32(1.0)
+        for j in range(3):
+            q = F32(1.0) if j == i else F32(F32(X[j] - x0) / F32(X[j] - X[i]))
+            w = F32(w * q)
+        L.append(w)
+    return L
+
+
+def _two_in_one_acc(samples):
+    """TwoinOne over samples, then average (mediaparas.cpp:838-914),
+    element-wise over arrays of points.  `samples` is a list of
+    (vp0, vs0, dc0, vp1, vs1, dc1) arrays (upper and lower side of each
+    sample).  The reference's test `nsamp>0 && muval==0` compares pointers
+    and is never true, so mu's harmonic sum restarts at 0 after any fluid
+    sample.  Returns (density, mu, lambda)."""
+    f64 = np.float64
+    shape = np.shape(samples[0][0])
+    rhoval, lamval, muval = np.zeros(shape, F32), np.zeros(shape, F32), np.zeros(shape, F32)
+    for vp0, vs0, dc0, vp1, vs1, dc1 in samples:
+        m0 = (dc0 * vs0) * vs0
+        m1 = (dc1 * vs1) * vs1
+        l0 = (((vp0 * vp0) * dc0).astype(f64) - 2.0 * m0.astype(f64)).astype(F32)
+        l1 = (((vp1 * vp1) * dc1).astype(f64) - 2.0 * m1.astype(f64)).astype(F32)
+        rhoval = (rhoval.astype(f64) + 0.5 * (dc0 + dc1).astype(f64)).astype(F32)
+        fluid = (m0 < SEIS_ZERO) | (m1 < SEIS_ZERO)
+        with np.errstate(divide="ignore"):
+            acc = ((muval.astype(f64) + 0.5 / m0.astype(f64)) + 0.5 / m1.astype(f64)).astype(F32)
+            lamval = ((lamval.astype(f64) + 0.5 / l0.astype(f64)) + 0.5 / l1.astype(f64)).astype(F32)
+        muval = np.where(fluid, F32(0.0), acc).astype(F32)
+    n = F32(len(samples))
+    density = (rhoval / n).astype(F32)
+    with np.errstate(divide="ignore"):
+        mu = np.where(muval > 0, (f64(n) * 1.0 / muval.astype(f64)), 0.0).astype(F32)
+        lamb = (f64(n) * 1.0 / lamval.astype(f64)).astype(F32)
+    return density, mu, lamb
+
+
+def _finish_media(density, miu, lamb, vel3d, x, y, z, smooth, sample_pairs):
+    """ApplySmooth (mediaparas.cpp:1712-1824) when a sampling factor is above
+    1, MediaStatistics (1837-1905) and the ghost extension of readdata.
+    `smooth` = (sampx, sampy, sampz, crit_value, crit_perc);
+    sample_pairs(x0, y0, z0) gives the (vp0, vs0, dc0, vp1, vs1, dc1) arrays
+    of the upper and lower side at arrays of points."""
+    ni, nj, nk = (s - 2 * LFD for s in x.shape)
+    v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj), slice(LFD, LFD + nk))
+    if smooth is not None and max(smooth[:3]) > 1:
+        sampx, sampy, sampz, cv, cp = smooth
+
+        def over(a, b):
+            d = np.abs(a - b)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                return (d >= cv) | (d / np.minimum(a, b) >= cp)
+
+        c = vel3d[v]
+        ns = []
+        for ax, samp in enumerate((sampx, sampy, sampz)):
+            lo = [slice(LFD, LFD + ni), slice(LFD, LFD + nj), slice(LFD, LFD + nk)]
+            hi = list(lo)
+            lo[ax] = slice(LFD - 1, LFD - 1 + (ni, nj, nk)[ax])
+            hi[ax] = slice(LFD + 1, LFD + 1 + (ni, nj, nk)[ax])
+            ns.append(np.where(over(c, vel3d[tuple(lo)]) | over(c, vel3d[tuple(hi)]), samp, 1))
+        flagged = (ns[0] > 1) | (ns[1] > 1) | (ns[2] > 1)
+        counts = np.stack([n_[flagged] for n_ in ns], 1)
+        cells = np.argwhere(flagged) + LFD
+        for key in {tuple(r) for r in counts.tolist()}:
+            sel = cells[(counts == key).all(1)]
+            I, J, K = sel[:, 0], sel[:, 1], sel[:, 2]
+            nx, ny, nz = key
+            samples = []
+            for mi in range(1, nx + 1):
+                for mj in range(1, ny + 1):
+                    for mk in range(1, nz + 1):
+                        lx = _lag3(F32(1.0 + 2.0 / nx * (mi - 1 + 0.5)))
+                        ly = _lag3(F32(1.0 + 2.0 / ny * (mj - 1 + 0.5)))
+                        lz = _lag3(F32(1.0 + 2.0 / nz * (mk - 1 + 0.5)))
+                        pos = []
+                        for crd in (x, y, z):  # interp3d over the 3x3x3 neighbourhood, float
+                            acc = np.zeros(len(sel), F32)
+                            for a in range(3):
+                                for b in range(3):
+                                    for g in range(3):
+                                        w = F32(F32(lx[a] * ly[b]) * lz[g])
+                                        acc = acc + w * crd[I - 1 + a, J - 1 + b, K - 1 + g]
+                            pos.append(acc)
+                        samples.append(sample_pairs(*pos))
+            density[I, J, K], miu[I, J, K], lamb[I, J, K] = _two_in_one_acc(samples)
     shape = x.shape
     alpha, beta, rho = np.zeros(shape, F32), np.zeros(shape, F32), np.zeros(shape, F32)
-    beta[v] = np.sqrt(miu / density)
-    alpha[v] = np.sqrt((lamb.astype(np.float64) + 2.0 * miu.astype(np.float64)) / density).astype(F32)
-    rho[v] = density
+    beta[v] = np.sqrt(miu[v] / density[v])
+    alpha[v] = np.sqrt((lamb[v].astype(np.float64) + 2.0 * miu[v].astype(np.float64)) / density[v]).astype(F32)
+    rho[v] = density[v]
     for a in (alpha, beta, rho):
         _sc.equivalence_extend(a)
     return alpha, beta, rho
+
+
+def volume_media(path, x, y, z, zwmedia=True, smooth=None):
+    """Media of a volume file on the grid (readdata, mediaparas.cpp:2140-2219):
+    volume_discrete at every point, get2in1, the optional smoothing, then
+    MediaStatistics and the ghost extension.  Returns (alpha, beta, rho)."""
+    vt = _volume_table(path, x, y, z, zwmedia)
+    Vp, Vs, Dc = _volume_sample(vt, x, y, z)
+    density, miu, lamb = _get2in1(Vp, Vs, Dc)
+    vel3d = _vel3d(Vp, Vp, *smooth[3:]) if smooth is not None else None
+
+    def sample_pairs(px, py, pz):
+        vp, vs, dc = _volume_sample(vt, px, py, pz)
+        return vp, vs, dc, vp, vs, dc
+
+    return _finish_media(density, miu, lamb, vel3d, x, y, z, smooth, sample_pairs)
 
 
 # ---------------------------------------------------------------------------
@@ -901,50 +1026,39 @@ def _i3d_point(t, col, z0, prev):
     return prev
 
 
-def _two_in_one(v0, v1):
-    """TwoinOne + average for one sample (mediaparas.cpp:838-914): density is
-    the mean, lambda and mu the harmonic means of the two sides (mu = 0 when
-    either side is fluid)."""
-    (vp0, vs0, dc0), (vp1, vs1, dc1) = v0, v1
-    miu = [F32(F32(dc * vs) * vs) for dc, vs in ((dc0, vs0), (dc1, vs1))]
-    lam = [F32(float(F32(F32(vp * vp) * dc)) - 2.0 * float(m)) for vp, dc, m in ((vp0, dc0, miu[0]), (vp1, dc1, miu[1]))]
-    rhoval = F32(0.0 + 0.5 * float(F32(dc0 + dc1)))
-    if miu[0] < SEIS_ZERO or miu[1] < SEIS_ZERO:
-        muval = F32(0.0)
-    else:
-        muval = F32(0.0 + 0.5 / float(miu[0]) + 0.5 / float(miu[1]))
-    lamval = F32(0.0 + 0.5 / float(lam[0]) + 0.5 / float(lam[1]))
-    density = rhoval
-    mu = F32(1.0 / float(muval)) if muval > 0 else F32(0.0)
-    lamb = F32(1.0 / float(lamval))
-    return density, mu, lamb
-
-
-def interface3d_media(path, nx, ny, x0, y0, dx, dy, nint, x, y, z):
-    """Media of an interface3D file on the grid: interface3d_discrete at each
-    valid cell, TwoinOne/average, MediaStatistics and the ghost extension.
-    Returns (alpha, beta, rho) with ghosts.  Pure Python per cell: meant for
-    grids up to a few million cells."""
+def interface3d_media(path, nx, ny, x0, y0, dx, dy, nint, x, y, z, smooth=None):
+    """Media of an interface3D file on the grid: interface3d_discrete at every
+    point in the reference's loop order (a point with no matching layer keeps
+    the previous point's values), TwoinOne/average, the optional smoothing,
+    MediaStatistics and the ghost extension.  Returns (alpha, beta, rho).
+    Pure Python per point: meant for grids up to a few million cells."""
     t = read_interface3d(path, nx, ny, x0, y0, dx, dy, nint)
-    ni, nj, nk = (s - 2 * LFD for s in x.shape)
     shape = x.shape
-    alpha, beta, rho = np.zeros(shape, F32), np.zeros(shape, F32), np.zeros(shape, F32)
+    side = [np.zeros(shape, F32) for _ in range(6)]   # vp0 vs0 dc0 vp1 vs1 dc1
     prev = ((F32(0.0),) * 3, (F32(0.0),) * 3)
-    for i in range(LFD, LFD + ni):
-        for j in range(LFD, LFD + nj):
-            cols = {}
-            for k in range(LFD, LFD + nk):
+    cols = {}
+    for i in range(shape[0]):
+        for j in range(shape[1]):
+            for k in range(shape[2]):
                 key = (x[i, j, k], y[i, j, k])
                 if key not in cols:
                     cols[key] = _i3d_column(t, *key)
                 prev = _i3d_point(t, cols[key], z[i, j, k], prev)
-                dc, mu, lamb = _two_in_one(*prev)
-                beta[i, j, k] = np.sqrt(F32(mu / dc))
-                alpha[i, j, k] = F32(math.sqrt((float(lamb) + 2.0 * float(mu)) / float(dc)))
-                rho[i, j, k] = dc
-    for a in (alpha, beta, rho):
-        _sc.equivalence_extend(a)
-    return alpha, beta, rho
+                for q in range(6):
+                    side[q][i, j, k] = prev[q // 3][q % 3]
+        cols.clear()
+    density, miu, lamb = _two_in_one_acc([tuple(side)])
+    vel3d = _vel3d(side[0], side[3], *smooth[3:]) if smooth is not None else None
+
+    def sample_pairs(px, py, pz):
+        out = [np.zeros(len(px), F32) for _ in range(6)]
+        for n in range(len(px)):
+            pr = _i3d_point(t, _i3d_column(t, px[n], py[n]), pz[n], prev)
+            for q in range(6):
+                out[q][n] = pr[q // 3][q % 3]
+        return tuple(out)
+
+    return _finish_media(density, miu, lamb, vel3d, x, y, z, smooth, sample_pairs)
 
 
 # ---------------------------------------------------------------------------
@@ -971,14 +1085,15 @@ def load(conf_path, seispath=None, media=None, zwmedia=True):
     x, y, z = vmap_coords(posx, posy, posz, gp, eq, nk)
     met = _sc.metrics_from_coords(x, y, z, m["steph"])
 
+    smooth = (*m["sampling"], m["crit_value"], m["crit_perc"])
     if media is not None:
         alpha, beta, rho = (np.asarray(a, F32) for a in media)
     elif m["media_type"] == "volume":
-        alpha, beta, rho = volume_media(m["media_filename"], x, y, z, zwmedia)
+        alpha, beta, rho = volume_media(m["media_filename"], x, y, z, zwmedia, smooth)
     elif m["media_type"] == "interface3D":
         c = ConfFile(conf_path, "main par")
         par = [c.int("interface3dpar", 3 + q) for q in range(7)]
-        alpha, beta, rho = interface3d_media(m["media_filename"], *par, x, y, z)
+        alpha, beta, rho = interface3d_media(m["media_filename"], *par, x, y, z, smooth)
     elif m["media_type"] == "interface":
         raise ConfError("media_type interface: the reference's readdata overwrites the interface media with zero "
                         "Vp/Vs/rho (get2in1, mediaparas.cpp:2163-2200) and MediaStatistics turns them into NaN; "

@@ -161,14 +161,14 @@ def i3d_plane(x, y):
     return -600.0 + 0.05 * x - 0.08 * y
 
 
-def write_interface3d(d):
+def write_interface3d(d, z2=-900.0, sampling=None, thresholds=None):
     """An interface3D media file (three interfaces: flat at +400 m with air
     above, a tilted plane, flat at -900 m) and a conf that points to it."""
     p = I3D
     X = p["x0"] + np.arange(p["nx"]) * p["dx"]
     Y = p["y0"] + np.arange(p["ny"]) * p["dy"]
     XX, YY = np.meshgrid(X, Y, indexing="ij")
-    zs = [np.full_like(XX, 400.0, dtype=float), i3d_plane(XX, YY), np.full_like(XX, -900.0, dtype=float)]
+    zs = [np.full_like(XX, 400.0, dtype=float), i3d_plane(XX, YY), np.full_like(XX, z2, dtype=float)]
     sides = [((0.0, 0.0, 0.0), ROCK_A_TOP), (ROCK_A_BOT, ROCK_B), (ROCK_B, ROCK_C)]
     out = []
     for n in range(3):
@@ -184,5 +184,12 @@ def write_interface3d(d):
     txt = conf.read_text().replace("media_type = volume", "media_type = interface3D").replace(
         "media_filename = vol.nc", "media_filename = media.int3d")
     txt += f"interface3dpar = {p['nx']} {p['ny']} {p['x0']} {p['y0']} {p['dx']} {p['dy']} 3\n"
+    if sampling is not None:
+        txt = txt.replace("sampling_point_per_cell = 0 0 0", "sampling_point_per_cell = " + " ".join(map(str, sampling)))
+    if thresholds is not None:
+        txt = txt.replace("threshold_velocity_jump_of_interface = 100",
+                          f"threshold_velocity_jump_of_interface = {thresholds[0]}")
+        txt = txt.replace("threshold_percent_change_of_interface = 0.1",
+                          f"threshold_percent_change_of_interface = {thresholds[1]}")
     conf.write_text(txt)
     return conf

@@ -396,3 +396,55 @@ def test_interface3d_reads_integer_geometry(tmp_path):
     assert t["flat"] == [True, False, True] and t["layer_const"] == [False, True, True]
     assert (t["indexZ"][:, :, :, 1] == np.arange(1, 4)[:, None, None]).all()
     assert (t["indexZ"][0, :, :, 0] == -1).all()       # air above the top interface
+
+
+def test_smoothing_mixes_the_row_that_straddles_an_interface(tmp_path):
+    # interface 2 at -980 m, between grid rows -900 and -1000 (100 m apart):
+    # with 2 samples in z, a flagged cell samples z -/+ 25 m, so only the
+    # -1000 m row mixes rock B and rock C; the samples of the -900 m row
+    # both lie in rock B
+    FX.write_set(tmp_path / "s", media=False)
+    sm = S.load(str(FX.write_interface3d(tmp_path / "s", z2=-980.0, sampling=(2, 2, 2))))
+    FX.write_set(tmp_path / "n", media=False)
+    un = S.load(str(FX.write_interface3d(tmp_path / "n", z2=-980.0)))
+    z = sm["coords"][2].astype(np.float64)
+    v = (slice(3, -3),) * 3
+    Z = z[v]
+    row = np.isclose(Z, -1000.0, atol=1e-3)
+    assert row.sum() == FX.NI * FX.NJ
+    muB, laB, dB = _moduli(FX.ROCK_B)
+    muC, laC, dC = _moduli(FX.ROCK_C)
+    mu, la, dc = 2 / (1 / muB + 1 / muC), 2 / (1 / laB + 1 / laC), (dB + dC) / 2
+    a, b, r = (sm[k][v].astype(np.float64) for k in ("alpha", "beta", "rho"))
+    np.testing.assert_allclose(r[row], dc, rtol=1e-6)
+    np.testing.assert_allclose(b[row], np.sqrt(mu / dc), rtol=2e-6)
+    np.testing.assert_allclose(a[row], np.sqrt((la + 2 * mu) / dc), rtol=2e-6)
+    # unsmoothed, that row is rock C; rows far from both interfaces are unchanged
+    np.testing.assert_allclose(un["alpha"][v][row], FX.ROCK_C[0], rtol=2e-6)
+    far = (Z < -1050.0) | np.isclose(Z, -900.0, atol=1e-3)
+    for k in ("alpha", "beta", "rho"):
+        np.testing.assert_array_equal(sm[k][v][far], un[k][v][far])
+
+
+def test_smoothing_below_thresholds_changes_nothing(tmp_path):
+    FX.write_set(tmp_path / "a")
+    base = S.load(str(tmp_path / "a" / "SeisFD3D.conf"))
+    FX.write_set(tmp_path / "b")
+    conf = tmp_path / "b" / "SeisFD3D.conf"
+    conf.write_text(conf.read_text().replace("sampling_point_per_cell = 0 0 0", "sampling_point_per_cell = 3 3 3")
+                    .replace("threshold_velocity_jump_of_interface = 100", "threshold_velocity_jump_of_interface = 1e9"))
+    sm = S.load(str(conf))
+    for k in ("alpha", "beta", "rho"):
+        np.testing.assert_array_equal(sm[k], base[k])
+    # with every cell flagged, a linear volume stays linear to second order;
+    # the bottom row is left out: read_volume_new cuts the volume at the
+    # deepest grid z (a volume node here), so samples below it take that
+    # node's value
+    conf.write_text(conf.read_text().replace("threshold_velocity_jump_of_interface = 1e9",
+                                             "threshold_velocity_jump_of_interface = 0"))
+    sm = S.load(str(conf))
+    v = (slice(3, -3), slice(3, -3), slice(4, -3))
+    np.testing.assert_allclose(sm["alpha"][v], base["alpha"][v], rtol=5e-4)
+    assert not np.array_equal(sm["alpha"][v], base["alpha"][v])
+    bottom = (slice(3, -3), slice(3, -3), 3)
+    assert (sm["alpha"][bottom] < base["alpha"][bottom]).all()

@@ -185,6 +185,19 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
     if (!dims || dims->ni < 1 || dims->nj < 1 || dims->nk < 4 || dims->nt < 1 || !(dims->dh > 0) ||
         !(dims->dt > 0) || (precision != CGFD_FP32 && precision != CGFD_FP64))
         return CGFD_EINVAL;
+    {
+        // The stage kernels address a field with 32-bit byte offsets from its
+        // base pointer, so one padded field must stay below 4 GiB (e.g. FP32
+        // up to about 1000^3 cells).  Larger domains must be split.
+        const long nxp = round_up(dims->ni + 3, 64), plane = nxp * (dims->nj + 3L);
+        const unsigned long bytes = (unsigned long)round_up((dims->nk + 8L) * plane, 64) *
+                                    (precision == CGFD_FP64 ? 8 : 4);
+        if (bytes > 0xffffffffUL) {
+            fprintf(stderr, "cgfd_create: a padded field of %lu bytes exceeds the 4 GiB offset range of the stage "
+                            "kernels; split the domain\n", bytes);
+            return CGFD_EINVAL;
+        }
+    }
     cgfd_ctx *c = new cgfd_ctx();
     c->prec = precision;
     c->esz = precision == CGFD_FP64 ? 8 : 4;

@@ -57,7 +57,10 @@ typedef struct cgfd_dims {
 } cgfd_dims;
 
 /* ChildProcs::ChildProcs (calculate.cu:107-1003): allocate device state on
- * `device` (HIP ordinal).  precision: CGFD_FP32 or CGFD_FP64. */
+ * `device` (HIP ordinal).  precision: CGFD_FP32 or CGFD_FP64.  Returns
+ * CGFD_EINVAL (before touching the GPU) when one padded field, about
+ * (ni+67)*(nj+3)*(nk+8) elements, would exceed 4 GiB: the stage kernels
+ * address a field with 32-bit byte offsets. */
 int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out);
 /* ChildProcs::~ChildProcs (calculate.cu:1016-1334) */
 void cgfd_destroy(cgfd_ctx *ctx);

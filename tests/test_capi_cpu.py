@@ -49,4 +49,11 @@ def test_create_rejects_bad_dims_without_touching_the_gpu():
     assert lib.cgfd_create(ctypes.byref(bad), 0, 0, ctypes.byref(h)) == -1
     bad = cgfd3d.Dims(10, 10, 10, 100.0, 0.01, 10)
     assert lib.cgfd_create(ctypes.byref(bad), 7, 0, ctypes.byref(h)) == -1
+    # one padded field past the 4 GiB range of the kernels' 32-bit offsets:
+    # 1024 x 1024 x 1000 cells fit in HBM but not in one field's offsets
+    big = cgfd3d.Dims(1024, 1024, 1000, 100.0, 0.01, 10)
+    assert lib.cgfd_create(ctypes.byref(big), 0, 0, ctypes.byref(h)) == -1
+    # FP64 halves the cells that fit: a 900^3 field is 3.2 GB in FP32, 6.3 GB in FP64
+    mid = cgfd3d.Dims(900, 900, 900, 100.0, 0.01, 10)
+    assert lib.cgfd_create(ctypes.byref(mid), 1, 0, ctypes.byref(h)) == -1
     assert not h.value

@@ -10,11 +10,14 @@ are synthetic (seeded) and resident in HBM before timing starts.
 
 A step is one full RK4 time step (4 stages) of the whole grid.  The metric
 is Gcell-RKstages/s = ni*nj*nk*4*steps / t.  The roofline entry divides the
-algorithmic bytes of one stage (192 B per cell, 168 B when the horizontal
-metrics are the constants the mesh implies, + 144 B per PML cell-axis, FP32)
-by the mean duration of the fused stage kernel measured with HIP events on
-the engine's stream over the timed steps; `traffic` is the measured HBM bytes
-per launch from the committed counter summary.  The CPU baseline is the oracle
+algorithmic bytes of one stage -- the compulsory traffic of the 3-register
+RK4 with 9 wavefield, 9 metric and 3 media values: 192 B per cell + 144 B per
+PML cell-axis, FP32 -- by the mean duration of the fused stage kernel
+measured with HIP events on the engine's stream over the timed steps.  When
+the horizontal metrics are the constants the mesh implies, the kernel loads
+only the 3 zeta metrics (168 B per cell); `config.bytes_per_stage_loaded`
+reports that figure.  `traffic` is the measured HBM bytes per launch from the
+committed counter summary.  The CPU baseline is the oracle
 (a C/OpenMP restatement of the reference kernels) timed on this host on three
 steps (about 15 s of CPU work) of the same problem.
 """
@@ -166,6 +169,16 @@ def copy_calibration(mib, reps=3):
     log(f"copy calibration: {reps} x {mib} MiB device copy")
 
 
+def algorithmic_bytes(sc):
+    """Bytes of one RK stage by the per-unit model: 192 B per valid cell
+    (mean over the 4 stages of 39/57/57/39 floats) + 144 B per cell and per
+    active PML axis (mean of 27/45/45/27 floats), FP32."""
+    ni, nj, nk = sc["ni"], sc["nj"], sc["nk"]
+    nab = [int(v) for v in sc["nabs"]]
+    npml = (nab[0] + nab[1]) * nj * nk + (nab[2] + nab[3]) * ni * nk + (nab[4] + nab[5]) * ni * nj
+    return 192.0 * ni * nj * nk + 144.0 * npml
+
+
 def measured_traffic(workload, variant):
     """HBM bytes per stage-kernel launch for this workload, from the committed
     counter summary (profiles/stage_traffic.json), or None."""
@@ -234,7 +247,8 @@ def main():
     nv = sc["ni"] * sc["nj"] * sc["nk"]
     value = nv * 4 * args.steps / t / 1e9
     stage_ms = prof["stage_ms"] / max(prof["stage_n"], 1)
-    bytes_stage = sim.stage_bytes()
+    bytes_stage = algorithmic_bytes(sc)
+    bytes_loaded = sim.stage_bytes()
     achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
     seis = sim.get_seismo()
     finite = bool(abs(seis).max() < float("inf"))
@@ -251,7 +265,9 @@ def main():
 
     cfg = describe(sc)
     cfg.update({"stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
-                "bytes_per_stage_model": bytes_stage, "finite_output": finite})
+                "bytes_per_stage_model": bytes_stage, "bytes_per_stage_loaded": bytes_loaded,
+                "hbm_frac_of_loaded_bytes": round(bytes_loaded / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "finite_output": finite})
     line = {
         "metric": "Gcell-RKstages/s",
         "value": round(value, 4),

@@ -120,6 +120,61 @@ def batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def surrogate_bench(args):
+    """Learned surrogate with FD fallback on the fixed 80x80x48 Gaussian-hill
+    medium (600 steps, dt = 6 ms): FD training data from the batched engine,
+    two FNOs (width 48, 12 modes, 4 layers) trained on the MFMA kernels, the
+    gate calibrated for a 5 % median PGV error on a calibration set, then
+    the test scenarios answered by the gated surrogate.  Prints one JSON line
+    whose value is the gated scenarios/s, with sequential plain FD beside it."""
+    import numpy as np
+
+    import surrogate
+
+    base = surrogate.base_medium(ni=80, nj=80, nk=48, nt=600, dt=0.006, npml=10)
+    ptr = surrogate.draw_sources(args.train, base, seed=1)
+    pcal = surrogate.draw_sources(args.calib, base, seed=2)
+    pte = surrogate.draw_sources(args.test, base, seed=3)
+    nseq = min(8, len(pte))
+    _, t_seq = surrogate.timed(lambda: surrogate.run_fd(base, pte[:nseq], group=1))
+    log(f"sequential FD: {nseq} scenarios in {t_seq:.1f} s")
+    ytr, t_fd = surrogate.timed(lambda: surrogate.run_fd(base, ptr, group=32))
+    log(f"batched FD training data: {len(ptr)} scenarios in {t_fd:.1f} s")
+    ycal = surrogate.run_fd(base, pcal, group=32)
+    yte = surrogate.run_fd(base, pte, group=32)
+    s = surrogate.Surrogate(base, width=48, modes=12, layers=4)
+    losses, t_train = surrogate.timed(lambda: s.fit(ptr, ytr, epochs=args.epochs, batch=16, lr=3e-3, log=log))
+    thr = s.calibrate(pcal, ycal, target_err=0.05)
+    (pred, ind), t_sur = surrogate.timed(lambda: s.predict(pte))
+    (ans, ok), t_gate = surrogate.timed(lambda: s.answer(pte))
+    err_sur = surrogate.pgv_error(pred, yte)
+    err_pw = surrogate.pgv_pointwise_error(pred, yte)
+    err_gate = surrogate.pgv_error(ans, yte)
+    value = len(pte) / t_gate
+    seq_rate = nseq / t_seq
+    line = {
+        "metric": "scenarios/s", "value": round(value, 3), "unit": "scenarios/s", "n_gpus": 1,
+        "steps": base["nt"], "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (surrogate, fp32 accumulation); f32 (FD)", "data": "synthetic",
+        "config": {"workload": "surrogate-hill-80x80x48-nt600", "train_scenarios": len(ptr),
+                   "calibration_scenarios": len(pcal), "test_scenarios": len(pte), "epochs": args.epochs,
+                   "fno": {"width": 48, "modes": 12, "layers": 4, "ensemble": 2},
+                   "fd_sequential_scenarios_per_s": round(seq_rate, 3),
+                   "fd_batched_scenarios_per_s": round(len(ptr) / t_fd, 3),
+                   "surrogate_only_scenarios_per_s": round(len(pte) / t_sur, 1),
+                   "gated_speedup_vs_sequential_fd": round(value / seq_rate, 2),
+                   "train_seconds": round(t_train, 1), "final_train_loss": round(losses[-1], 5),
+                   "gate_target_median_pgv_error": 0.05, "gate_threshold": thr,
+                   "accepted_fraction": round(float(ok.mean()), 3),
+                   "median_pgv_error_surrogate_all": round(float(np.median(err_sur)), 4),
+                   "median_pointwise_pgv_error_surrogate_all": round(float(np.median(err_pw)), 4),
+                   "median_pgv_error_gated": round(float(np.median(err_gate)), 4),
+                   "pgv_error": "relative L2 of the three surface PGV maps per scenario vs full FD"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(sc, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -197,8 +252,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat", "batch"])
+    ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat", "batch", "surrogate"])
     ap.add_argument("--batch", type=int, default=16, help="scenarios of the batch workload")
+    ap.add_argument("--train", type=int, default=512, help="training scenarios of the surrogate workload")
+    ap.add_argument("--calib", type=int, default=32, help="gate-calibration scenarios of the surrogate workload")
+    ap.add_argument("--test", type=int, default=64, help="test scenarios of the surrogate workload")
+    ap.add_argument("--epochs", type=int, default=60, help="training epochs of the surrogate workload")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
@@ -214,6 +273,9 @@ def main():
 
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.workload == "surrogate":
+        surrogate_bench(args)
         return
 
     import cgfd3d

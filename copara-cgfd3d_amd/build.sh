@@ -18,6 +18,10 @@ hipcc -shared -fPIC --offload-arch="$ARCH" -o "$here/libcgfd3d.so.tmp" "$here/_b
       "$here/_build/cgfd_ncio.o"
 mv -f "$here/libcgfd3d.so.tmp" "$here/libcgfd3d.so"
 echo "built $here/libcgfd3d.so"
+# the surrogate's BF16 MFMA kernels
+hipcc "${FLAGS[@]}" -shared "$here/csrc/fno_kernels.hip" -o "$here/libcgfd_fno.so.tmp"
+mv -f "$here/libcgfd_fno.so.tmp" "$here/libcgfd_fno.so"
+echo "built $here/libcgfd_fno.so"
 # the C++ host driver (one child-rank run through the C-ABI)
 g++ -O2 -std=c++17 -Wall -I"$root/include" "$here/host/cgfd_child.cpp" -L"$here" -lcgfd3d \
     -Wl,-rpath,'$ORIGIN' -o "$here/cgfd_child"

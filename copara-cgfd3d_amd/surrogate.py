@@ -1,0 +1,401 @@
+"""Fourier-neural-operator surrogate for surface peak ground velocity, with
+the finite-difference engine as its data source, fallback and judge.
+
+For a fixed topography and medium, the surrogate maps a source
+(x, y, depth, strike, dip, rake, log10 M0) to the three surface PGV maps the
+engine's PGV capture produces.
+
+- **Linearity in M0.** The engine is linear in the moment, so the network
+  predicts log10(PGV / M0) and log10 M0 is added exactly.
+- **Matrix products.** Every product of the network runs on the BF16 MFMA
+  kernel of libcgfd_fno.so (FP32 accumulation), in the forward pass and in
+  both gradients of the backward pass: the lifting and projection layers,
+  the pointwise channel mixing and the per-mode complex spectral mixing
+  (realified as 2x2 real blocks). PyTorch supplies device memory, the FFTs,
+  GELU and the optimizer.
+- **Gate.** Two networks trained from different seeds. Their disagreement is
+  the error indicator, and scenarios above the threshold are re-run by the
+  FD engine.
+
+There is no reference counterpart (the reference has no learned model).
+"""
+import ctypes
+import math
+import os
+import time
+
+import numpy as np
+import torch
+
+import scenario
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class SurrogateError(RuntimeError):
+    pass
+
+
+def load_library():
+    """libcgfd_fno.so; there is no fallback when it is missing."""
+    global _LIB
+    if _LIB is None:
+        path = os.environ.get("CGFD_FNO_LIB", os.path.join(_HERE, "libcgfd_fno.so"))
+        if not os.path.exists(path):
+            raise SurrogateError(f"{path} is not built (run copara-cgfd3d_amd/build.sh)")
+        lib = ctypes.CDLL(path)
+        L, P, I = ctypes.c_long, ctypes.c_void_p, ctypes.c_int
+        lib.cgfd_fno_gemm_bf16.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P]
+        lib.cgfd_fno_gemm_bf16.restype = I
+        _LIB = lib
+    return _LIB
+
+
+def _span_ok(t, offsets):
+    """Every element the kernel addresses from t's data pointer lies inside
+    t's storage (all strides here are non-negative)."""
+    last = t.storage_offset() + sum(offsets)
+    return t.storage_offset() >= 0 and last < t.untyped_storage().nbytes() // t.element_size()
+
+
+def gemm(a, b, out, accumulate=False):
+    """out[z] (+)= a[z] @ b[z] on the matrix cores.  a: (batch,) M x K bf16,
+    b: (batch,) K x N bf16, out: (batch,) M x N fp32 with unit column stride;
+    any non-negative strides.  Runs on the current stream."""
+    if a.dim() == 2:
+        a, b, out = a.unsqueeze(0), b.unsqueeze(0), out.unsqueeze(0)
+    if not (a.is_cuda and b.is_cuda and out.is_cuda):
+        raise SurrogateError("gemm operands must be on the GPU")
+    if a.dtype != torch.bfloat16 or b.dtype != torch.bfloat16 or out.dtype != torch.float32:
+        raise SurrogateError("gemm takes bf16 inputs and an fp32 output")
+    Z, M, K = a.shape
+    Zb, Kb, N = b.shape
+    if (Zb, Kb) != (Z, K) or tuple(out.shape) != (Z, M, N) or out.stride(2) != 1:
+        raise SurrogateError(f"gemm shapes {tuple(a.shape)} x {tuple(b.shape)} -> {tuple(out.shape)}")
+    if min(a.stride() + b.stride() + out.stride()) < 0:
+        raise SurrogateError("gemm strides must be non-negative")
+    for t, ext in ((a, (Z, M, K)), (b, (Z, K, N)), (out, (Z, M, N))):
+        if not _span_ok(t, [(e - 1) * s for e, s in zip(ext, t.stride())]):
+            raise SurrogateError("gemm operand strides address memory outside the tensor")
+    sa, sb, so = a.stride(), b.stride(), out.stride()
+    rc = load_library().cgfd_fno_gemm_bf16(
+        a.data_ptr(), sa[0], sa[1], sa[2], b.data_ptr(), sb[0], sb[1], sb[2], out.data_ptr(), so[0], so[1],
+        M, N, K, Z, int(accumulate), torch.cuda.current_stream().cuda_stream)
+    if rc != 0:
+        raise SurrogateError(f"cgfd_fno_gemm_bf16 returned {rc}")
+    return out
+
+
+def _weight_grad(xb, gyb, chunk=4096):
+    """x^T @ gy for tall x [R, K] and gy [R, N]: R is split into chunks (the
+    batch index of the GEMM) whose partial products are summed."""
+    R, K = xb.shape
+    N = gyb.shape[1]
+    full = R // chunk
+    gw = torch.zeros(K, N, device=xb.device, dtype=torch.float32)
+    if full:
+        xa = xb[: full * chunk].view(full, chunk, K).transpose(1, 2)     # [full, K, chunk]
+        ga = gyb[: full * chunk].view(full, chunk, N)                    # [full, chunk, N]
+        part = torch.empty(full, K, N, device=xb.device, dtype=torch.float32)
+        gemm(xa, ga, part)
+        gw += part.sum(0)
+    if R - full * chunk:
+        gemm(xb[full * chunk:].t(), gyb[full * chunk:], gw, accumulate=True)
+    return gw
+
+
+class _Matmul(torch.autograd.Function):
+    """y = x @ w with x [R, K], w [K, N] (fp32 masters, bf16 on the cores)."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        xb = x.contiguous().to(torch.bfloat16)
+        wb = w.contiguous().to(torch.bfloat16)
+        y = torch.empty(x.shape[0], w.shape[1], device=x.device, dtype=torch.float32)
+        gemm(xb, wb, y)
+        ctx.save_for_backward(xb, wb)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        xb, wb = ctx.saved_tensors
+        gyb = gy.contiguous().to(torch.bfloat16)
+        gx = torch.empty(xb.shape, device=gy.device, dtype=torch.float32)
+        gemm(gyb, wb.t(), gx)
+        return gx, _weight_grad(xb, gyb)
+
+
+class _Bmm(torch.autograd.Function):
+    """y[p] = x[p] @ w[p] with x [P, B, K], w [P, K, N]."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        xb = x.contiguous().to(torch.bfloat16)
+        wb = w.contiguous().to(torch.bfloat16)
+        y = torch.empty(x.shape[0], x.shape[1], w.shape[2], device=x.device, dtype=torch.float32)
+        gemm(xb, wb, y)
+        ctx.save_for_backward(xb, wb)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        xb, wb = ctx.saved_tensors
+        gyb = gy.contiguous().to(torch.bfloat16)
+        gx = torch.empty(xb.shape, device=gy.device, dtype=torch.float32)
+        gemm(gyb, wb.transpose(1, 2), gx)
+        gw = torch.empty(wb.shape, device=gy.device, dtype=torch.float32)
+        gemm(xb.transpose(1, 2), gyb, gw)
+        return gx, gw
+
+
+def matmul(x, w, backend):
+    return _Matmul.apply(x, w) if backend == "mfma" else x @ w
+
+
+def bmm(x, w, backend):
+    return _Bmm.apply(x, w) if backend == "mfma" else torch.bmm(x, w)
+
+
+# ---------------------------------------------------------------------------
+# the network
+# ---------------------------------------------------------------------------
+class FNO2d(torch.nn.Module):
+    """Lifting, `layers` Fourier layers (spectral mixing of the lowest
+    `modes` x `modes` frequencies + pointwise mixing, GELU between layers)
+    and a two-layer projection, on channels-last [B, H, W, C] activations.
+    backend "mfma" runs the products on libcgfd_fno.so; "torch" is the plain
+    FP32 reference used to check gradients."""
+
+    def __init__(self, cin, cout, width=64, modes=16, layers=4, proj=128, seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+
+        def lin(i, o):
+            return torch.nn.Parameter((torch.rand(i, o, generator=g) * 2 - 1) / math.sqrt(i))
+
+        self.width, self.modes, self.layers = width, modes, layers
+        self.lift_w, self.lift_b = lin(cin, width), torch.nn.Parameter(torch.zeros(width))
+        P = 2 * modes * modes
+        scale = 1.0 / (width * width)
+        self.spec_re = torch.nn.ParameterList(
+            [torch.nn.Parameter(scale * torch.rand(P, width, width, generator=g)) for _ in range(layers)])
+        self.spec_im = torch.nn.ParameterList(
+            [torch.nn.Parameter(scale * torch.rand(P, width, width, generator=g)) for _ in range(layers)])
+        self.pw_w = torch.nn.ParameterList([lin(width, width) for _ in range(layers)])
+        self.pw_b = torch.nn.ParameterList([torch.nn.Parameter(torch.zeros(width)) for _ in range(layers)])
+        self.p1_w, self.p1_b = lin(width, proj), torch.nn.Parameter(torch.zeros(proj))
+        self.p2_w, self.p2_b = lin(proj, cout), torch.nn.Parameter(torch.zeros(cout))
+
+    def _spectral(self, v, l, backend):
+        B, H, W, C = v.shape
+        m = self.modes
+        ft = torch.fft.rfft2(v, dim=(1, 2))                                   # [B, H, W/2+1, C]
+        sel = torch.cat([ft[:, :m, :m], ft[:, H - m:, :m]], dim=1)            # [B, 2m, m, C]
+        x = sel.permute(1, 2, 0, 3).reshape(2 * m * m, B, C)                  # [P, B, C]
+        xr = torch.cat([x.real, x.imag], dim=2)                               # [P, B, 2C]
+        wre, wim = self.spec_re[l], self.spec_im[l]
+        wr = torch.cat([torch.cat([wre, wim], 2), torch.cat([-wim, wre], 2)], 1)   # [P, 2C, 2C]
+        yr = bmm(xr, wr, backend)
+        y = torch.complex(yr[..., :C], yr[..., C:]).reshape(2 * m, m, B, C).permute(2, 0, 1, 3)
+        out = torch.zeros(B, H, W // 2 + 1, C, dtype=ft.dtype, device=v.device)
+        out[:, :m, :m] = y[:, :m]
+        out[:, H - m:, :m] = y[:, m:]
+        return torch.fft.irfft2(out, s=(H, W), dim=(1, 2))
+
+    def forward(self, x, backend="mfma"):
+        B, H, W, _ = x.shape
+        R = B * H * W
+        v = (matmul(x.reshape(R, -1), self.lift_w, backend) + self.lift_b).reshape(B, H, W, self.width)
+        for l in range(self.layers):
+            pw = (matmul(v.reshape(R, self.width), self.pw_w[l], backend) + self.pw_b[l]).reshape(v.shape)
+            v = self._spectral(v, l, backend) + pw
+            if l < self.layers - 1:
+                v = torch.nn.functional.gelu(v)
+        h = torch.nn.functional.gelu(matmul(v.reshape(R, self.width), self.p1_w, backend) + self.p1_b)
+        return (matmul(h, self.p2_w, backend) + self.p2_b).reshape(B, H, W, -1)
+
+
+# ---------------------------------------------------------------------------
+# scenarios and input encoding
+# ---------------------------------------------------------------------------
+FC = 1.5  # Ricker dominant frequency of every scenario (Hz)
+
+
+def base_medium(ni=80, nj=80, nk=48, nt=600, dt=0.006, npml=10):
+    """The fixed Gaussian-hill medium (h = 1.5 km, sigma = 3 km, three flat
+    layers) on which the surrogate is trained."""
+    sc = scenario.config_hill(ni=ni, nj=nj, nk=nk, nt=nt, dh=100.0, dt=dt, npml=npml, nrecv_side=2)
+    sc["recv_locs"] = sc["recv_locs"][:0]
+    sc["stf"] = scenario.stf_table("Ricker", nt, dt, 1.2 / FC, FC)[None, :]
+    return sc
+
+
+def draw_sources(n, base, seed):
+    """n sources on the base grid: grid-aligned (x, y) clear of the PML, 10
+    to 35 rows below the surface row (and above the bottom PML), uniform
+    strike/dip/rake, log-uniform M0 1e14-1e17."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk, dh = base["ni"], base["nj"], base["nk"], base["dh"]
+    npml = int(base["nabs"][0])
+    out = []
+    for _ in range(n):
+        i = int(rng.integers(npml + 4, ni - npml - 4))
+        j = int(rng.integers(npml + 4, nj - npml - 4))
+        dk = int(rng.integers(10, min(36, nk - npml - 2)))
+        s, d, r = rng.uniform([0.0, 10.0, -180.0], [360.0, 90.0, 180.0])
+        lm0 = rng.uniform(14.0, 17.0)
+        out.append(np.array([i * dh, j * dh, dk * dh, s, d, r, lm0]))
+    return np.array(out)
+
+
+def fd_scenario(base, p):
+    """The FD scenario dict of source parameters p."""
+    dh = base["dh"]
+    i, j, dk = int(round(p[0] / dh)), int(round(p[1] / dh)), int(round(p[2] / dh))
+    sc = dict(base)
+    sc["moment_locs"] = np.array([[i + 3, j + 3, base["nk"] - 1 - dk + 3]], np.int32)
+    sc["moment_vec"] = (scenario.angle2moment(p[3], p[4], p[5]) * 10 ** p[6]).astype(np.float32)[None, :]
+    sc["moment_stf"] = base["stf"]
+    return sc
+
+
+def run_fd(base, params, group=32, sims=None):
+    """Surface PGV maps [n, 3, ni, nj] of the scenarios, advanced `group` at a
+    time with cgfd_run_batch."""
+    import cgfd3d
+
+    out = np.empty((len(params), 3, base["ni"], base["nj"]), np.float32)
+    for g0 in range(0, len(params), group):
+        batch = []
+        for p in params[g0:g0 + group]:
+            sc = fd_scenario(base, p)
+            s = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+            s.setup(sc)
+            batch.append(s)
+        cgfd3d.run_batch(batch, base["nt"])
+        for k, s in enumerate(batch):
+            out[g0 + k] = s.get_pgv()[:, 3:-3, 3:-3]
+            s.close()
+    return out
+
+
+def encode(base, params):
+    """Input fields [n, ni, nj, 12] on the surface grid: x and y, the source's
+    relative position and a 1 km bump around its epicentre, depth, hypocentral
+    distance, and the six unit moment-tensor components."""
+    ni, nj, dh = base["ni"], base["nj"], base["dh"]
+    L = max(ni, nj) * dh
+    X, Y = np.meshgrid(np.arange(ni) * dh, np.arange(nj) * dh, indexing="ij")
+    out = np.empty((len(params), ni, nj, 12), np.float32)
+    for n, p in enumerate(params):
+        dx, dy = (X - p[0]) / L, (Y - p[1]) / L
+        dz = p[2] / L
+        m = scenario.angle2moment(p[3], p[4], p[5])
+        out[n, ..., 0], out[n, ..., 1] = X / L, Y / L
+        out[n, ..., 2], out[n, ..., 3] = dx, dy
+        out[n, ..., 4] = np.exp(-((X - p[0]) ** 2 + (Y - p[1]) ** 2) / (2 * 1000.0 ** 2))
+        out[n, ..., 5] = dz
+        out[n, ..., 6] = np.sqrt(dx * dx + dy * dy + dz * dz)
+        out[n, ..., 7:12] = m[[0, 1, 3, 4, 5]]   # mzz = -(mxx + myy) adds nothing
+    return out
+
+
+def target(pgv, params):
+    """log10(PGV / M0) [n, ni, nj, 3] (a floor keeps silent points finite)."""
+    m0 = 10.0 ** params[:, 6]
+    return np.log10(np.maximum(pgv / m0[:, None, None, None], 1e-30)).transpose(0, 2, 3, 1).astype(np.float32)
+
+
+def pgv_error(pred, ref):
+    """Relative L2 error of each scenario's three PGV maps: [n]."""
+    d = (pred - ref).reshape(len(ref), -1)
+    return np.linalg.norm(d, axis=1) / np.linalg.norm(ref.reshape(len(ref), -1), axis=1)
+
+
+def pgv_pointwise_error(pred, ref):
+    """Median over the surface points and components of |pred/ref - 1|, per
+    scenario: [n]."""
+    r = np.abs(pred / np.maximum(ref, 1e-30) - 1.0).reshape(len(ref), -1)
+    return np.median(r, axis=1)
+
+
+class Surrogate:
+    """Two FNOs from different seeds; predictions are their mean, and their
+    disagreement is the gate's error indicator."""
+
+    def __init__(self, base, width=64, modes=16, layers=4, seeds=(0, 1), device="cuda"):
+        self.base = base
+        self.nets = [FNO2d(12, 3, width, modes, layers, seed=s).to(device) for s in seeds]
+        self.device = device
+        self.mu = self.sd = None
+        self.threshold = None
+
+    def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None):
+        x = torch.from_numpy(encode(self.base, params)).to(self.device)
+        y = target(pgv, params)
+        self.mu, self.sd = float(y.mean()), float(y.std())
+        y = torch.from_numpy((y - self.mu) / self.sd).to(self.device)
+        losses = []
+        for k, net in enumerate(self.nets):
+            opt = torch.optim.Adam(net.parameters(), lr=lr)
+            sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, epochs)
+            g = torch.Generator().manual_seed(100 + k)
+            for ep in range(epochs):
+                perm = torch.randperm(len(x), generator=g)
+                tot = 0.0
+                for b0 in range(0, len(x), batch):
+                    idx = perm[b0:b0 + batch].to(self.device)
+                    loss = torch.mean((net(x[idx]) - y[idx]) ** 2)
+                    opt.zero_grad()
+                    loss.backward()
+                    opt.step()
+                    tot += loss.item() * len(idx)
+                sched.step()
+                losses.append(tot / len(x))
+                if log and (ep % 10 == 0 or ep == epochs - 1):
+                    log(f"net {k} epoch {ep}: loss {losses[-1]:.4f}")
+        return losses
+
+    @torch.no_grad()
+    def predict(self, params, batch=32):
+        """PGV maps [n, 3, ni, nj] (the ensemble mean) and the indicator [n]."""
+        x = torch.from_numpy(encode(self.base, params)).to(self.device)
+        outs = []
+        for net in self.nets:
+            outs.append(torch.cat([net(x[b:b + batch]) for b in range(0, len(x), batch)]))
+        m0 = torch.from_numpy(10.0 ** params[:, 6]).to(self.device).float()
+        pgv = [10.0 ** (o * self.sd + self.mu) * m0[:, None, None, None] for o in outs]
+        mean = sum(pgv) / len(pgv)
+        d = (pgv[0] - pgv[1]).reshape(len(x), -1)
+        ind = torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(mean.reshape(len(x), -1), dim=1)
+        return mean.permute(0, 3, 1, 2).cpu().numpy(), ind.cpu().numpy()
+
+    def calibrate(self, params, pgv, target_err=0.05):
+        """Largest indicator threshold whose accepted scenarios keep a median
+        PGV error at or below target_err on a calibration set."""
+        pred, ind = self.predict(params)
+        err = pgv_error(pred, pgv)
+        order = np.argsort(ind)
+        best = 0.0
+        for n in range(1, len(order) + 1):
+            if np.median(err[order[:n]]) <= target_err:
+                best = float(ind[order[n - 1]])
+        self.threshold = best
+        return best
+
+    def answer(self, params, fd_group=32):
+        """Gated answers: surrogate PGV where the indicator is below the
+        threshold, FD runs for the rest.  Returns (pgv, accepted mask)."""
+        pred, ind = self.predict(params)
+        ok = ind <= self.threshold
+        if (~ok).any():
+            pred[~ok] = run_fd(self.base, params[~ok], group=fd_group)
+        return pred, ok
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0

@@ -1,0 +1,122 @@
+"""The surrogate's BF16 MFMA kernel and network (surrogate.py).
+
+- The GEMM against numpy on bf16-rounded operands (products are exact in
+  FP32, so only the summation order differs): shapes off the 64x64x32 tile
+  grid, transposed strides, batches, accumulation.
+- Gradients of every parameter of the network, products on the matrix
+  cores in BF16, against the same network in FP32 on the CPU (torch
+  autograd with plain matmul), within the BF16 tolerance stated below.
+- A small training run on FD data from this engine: the trained network
+  beats the mechanism-blind baseline (the training set's mean
+  log10(PGV/M0) map plus log10 M0), and the gate's fallback answers come
+  from the FD engine.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+import surrogate  # noqa: E402
+
+
+def _bf16(a):
+    return torch.from_numpy(a).to(torch.bfloat16)
+
+
+def test_encoding_and_target_shapes():
+    base = {"ni": 20, "nj": 16, "dh": 100.0}
+    p = np.array([[500.0, 600.0, 1500.0, 30.0, 60.0, 90.0, 15.0]])
+    x = surrogate.encode(base, p)
+    assert x.shape == (1, 20, 16, 12)
+    assert x[0, 5, 6, 4] == pytest.approx(1.0)          # the epicentral bump peaks at the source
+    pgv = np.full((1, 3, 20, 16), 1e-3, np.float32)
+    t = surrogate.target(pgv, p)
+    np.testing.assert_allclose(t, -3.0 - 15.0, atol=1e-5)   # log10(PGV / M0)
+
+
+def test_library_missing_raises(monkeypatch):
+    monkeypatch.setattr(surrogate, "_LIB", None)
+    monkeypatch.setenv("CGFD_FNO_LIB", "/nonexistent/libcgfd_fno.so")
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.load_library()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,N,K,batch,ta,tb", [(64, 64, 32, 1, False, False), (70, 33, 45, 3, False, False),
+                                               (130, 96, 200, 2, True, False), (17, 129, 64, 4, False, True),
+                                               (1000, 48, 7, 1, True, True)])
+def test_gemm_matches_numpy(M, N, K, batch, ta, tb):
+    rng = np.random.default_rng(M + N + K)
+    a = rng.standard_normal((batch, M, K)).astype(np.float32)
+    b = rng.standard_normal((batch, K, N)).astype(np.float32)
+    A = _bf16(np.ascontiguousarray(a.transpose(0, 2, 1)) if ta else a).cuda()
+    B = _bf16(np.ascontiguousarray(b.transpose(0, 2, 1)) if tb else b).cuda()
+    A = A.transpose(1, 2) if ta else A
+    B = B.transpose(1, 2) if tb else B
+    out = torch.full((batch, M, N), 0.5, device="cuda")
+    surrogate.gemm(A, B, out)
+    ar = A.float().cpu().numpy().astype(np.float64)
+    br = B.float().cpu().numpy().astype(np.float64)
+    want = ar @ br
+    got = out.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+    surrogate.gemm(A, B, out, accumulate=True)
+    assert np.linalg.norm(out.cpu().numpy() - 2 * want) <= 1e-5 * np.linalg.norm(want)
+
+
+@pytest.mark.gpu
+def test_gemm_rejects_bad_operands():
+    a = torch.zeros(8, 8, dtype=torch.bfloat16, device="cuda")
+    b = torch.zeros(8, 6, dtype=torch.bfloat16, device="cuda")
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.gemm(a, b, torch.zeros(8, 8, device="cuda"))           # shape mismatch
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.gemm(a.float(), b, torch.zeros(8, 6, device="cuda"))   # not bf16
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.gemm(a, b, torch.zeros(6, 8, device="cuda").t())       # output columns not unit-stride
+
+
+@pytest.mark.gpu
+def test_gradients_match_fp32_cpu_reference():
+    # BF16 operands carry 8 significant bits: gradients of each parameter
+    # agree with the FP32 CPU network to a few percent in relative L2
+    net = surrogate.FNO2d(12, 3, width=32, modes=6, layers=4, proj=64, seed=3)
+    ref = surrogate.FNO2d(12, 3, width=32, modes=6, layers=4, proj=64, seed=3)
+    ref.load_state_dict(net.state_dict())
+    net = net.cuda()
+    rng = np.random.default_rng(1)
+    x = torch.from_numpy(rng.standard_normal((2, 24, 20, 12)).astype(np.float32))
+    y = torch.from_numpy(rng.standard_normal((2, 24, 20, 3)).astype(np.float32))
+    loss = torch.mean((net(x.cuda(), backend="mfma") - y.cuda()) ** 2)
+    loss.backward()
+    loss_ref = torch.mean((ref(x, backend="torch") - y) ** 2)
+    loss_ref.backward()
+    assert abs(float(loss) - float(loss_ref)) <= 2e-2 * float(loss_ref)
+    for (name, p), (_, q) in zip(net.named_parameters(), ref.named_parameters()):
+        g, h = p.grad.cpu(), q.grad
+        err = float(torch.linalg.vector_norm(g - h) / torch.linalg.vector_norm(h))
+        assert err < 5e-2, (name, err)
+
+
+@pytest.mark.gpu
+def test_trained_network_beats_baseline_and_gate_falls_back_to_fd():
+    base = surrogate.base_medium(ni=40, nj=40, nk=32, nt=260, dt=0.006, npml=8)
+    ptrain = surrogate.draw_sources(96, base, seed=1)
+    ptest = surrogate.draw_sources(16, base, seed=2)
+    print("FD data for the surrogate test", flush=True)
+    ytrain = surrogate.run_fd(base, ptrain, group=32)
+    ytest = surrogate.run_fd(base, ptest, group=16)
+    s = surrogate.Surrogate(base, width=24, modes=8, layers=4)
+    losses = s.fit(ptrain, ytrain, epochs=60, batch=8, lr=3e-3)
+    assert losses[59] < 0.3 * losses[0]
+    pred, ind = s.predict(ptest)
+    t = surrogate.target(ytrain, ptrain).mean(0)                 # mechanism-blind baseline
+    base_pred = np.stack([10.0 ** t.transpose(2, 0, 1) * 10.0 ** p[6] for p in ptest])
+    for metric in (surrogate.pgv_error, surrogate.pgv_pointwise_error):
+        e, b = np.median(metric(pred, ytest)), np.median(metric(base_pred, ytest))
+        print(f"{metric.__name__}: surrogate {e:.3f}, baseline {b:.3f}", flush=True)
+        assert e < 0.85 * b
+    # a threshold below every indicator sends everything to FD: exact answers
+    s.threshold = -1.0
+    got, ok = s.answer(ptest[:3], fd_group=3)
+    assert not ok.any()
+    np.testing.assert_array_equal(got, ytest[:3])

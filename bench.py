@@ -123,7 +123,7 @@ def batch_bench(args):
 def surrogate_bench(args):
     """Learned surrogate with FD fallback on the fixed 80x80x48 Gaussian-hill
     medium (600 steps, dt = 6 ms): FD training data from the batched engine,
-    two FNOs (width 48, 12 modes, 4 layers) trained on the MFMA kernels, the
+    two FNOs (4 layers; width and modes from the options) trained on the MFMA kernels, the
     gate calibrated for a 5 % median PGV error on a calibration set, then
     the test scenarios answered by the gated surrogate.  Prints one JSON line
     whose value is the gated scenarios/s, with sequential plain FD beside it."""
@@ -138,11 +138,11 @@ def surrogate_bench(args):
     nseq = min(8, len(pte))
     _, t_seq = surrogate.timed(lambda: surrogate.run_fd(base, pte[:nseq], group=1))
     log(f"sequential FD: {nseq} scenarios in {t_seq:.1f} s")
-    ytr, t_fd = surrogate.timed(lambda: surrogate.run_fd(base, ptr, group=32))
+    ytr, t_fd = surrogate.timed(lambda: surrogate.run_fd(base, ptr, group=32, log=log))
     log(f"batched FD training data: {len(ptr)} scenarios in {t_fd:.1f} s")
     ycal = surrogate.run_fd(base, pcal, group=32)
     yte = surrogate.run_fd(base, pte, group=32)
-    s = surrogate.Surrogate(base, width=48, modes=12, layers=4)
+    s = surrogate.Surrogate(base, width=args.fno_width, modes=args.fno_modes, layers=4)
     losses, t_train = surrogate.timed(lambda: s.fit(ptr, ytr, epochs=args.epochs, batch=16, lr=3e-3, log=log))
     thr = s.calibrate(pcal, ycal, target_err=0.05)
     (pred, ind), t_sur = surrogate.timed(lambda: s.predict(pte))
@@ -158,7 +158,7 @@ def surrogate_bench(args):
         "vs_baseline": None, "dtype": "bf16 (surrogate, fp32 accumulation); f32 (FD)", "data": "synthetic",
         "config": {"workload": "surrogate-hill-80x80x48-nt600", "train_scenarios": len(ptr),
                    "calibration_scenarios": len(pcal), "test_scenarios": len(pte), "epochs": args.epochs,
-                   "fno": {"width": 48, "modes": 12, "layers": 4, "ensemble": 2},
+                   "fno": {"width": args.fno_width, "modes": args.fno_modes, "layers": 4, "ensemble": 2},
                    "fd_sequential_scenarios_per_s": round(seq_rate, 3),
                    "fd_batched_scenarios_per_s": round(len(ptr) / t_fd, 3),
                    "surrogate_only_scenarios_per_s": round(len(pte) / t_sur, 1),
@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--calib", type=int, default=32, help="gate-calibration scenarios of the surrogate workload")
     ap.add_argument("--test", type=int, default=64, help="test scenarios of the surrogate workload")
     ap.add_argument("--epochs", type=int, default=60, help="training epochs of the surrogate workload")
+    ap.add_argument("--fno-width", type=int, default=48, help="FNO channel width of the surrogate workload")
+    ap.add_argument("--fno-modes", type=int, default=12, help="retained Fourier modes per axis")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)

@@ -260,9 +260,9 @@ def fd_scenario(base, p):
     return sc
 
 
-def run_fd(base, params, group=32, sims=None):
+def run_fd(base, params, group=32, log=None):
     """Surface PGV maps [n, 3, ni, nj] of the scenarios, advanced `group` at a
-    time with cgfd_run_batch."""
+    time with cgfd_run_batch; `log` gets a line per group."""
     import cgfd3d
 
     out = np.empty((len(params), 3, base["ni"], base["nj"]), np.float32)
@@ -277,6 +277,8 @@ def run_fd(base, params, group=32, sims=None):
         for k, s in enumerate(batch):
             out[g0 + k] = s.get_pgv()[:, 3:-3, 3:-3]
             s.close()
+        if log:
+            log(f"FD scenarios {g0 + len(batch)}/{len(params)}")
     return out
 
 

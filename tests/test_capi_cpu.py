@@ -1,5 +1,6 @@
-"""The C-ABI library loads on a machine without a GPU and exports every
-entry point that include/cgfd3d.h declares (no compute calls here)."""
+"""The C-ABI libraries load on a machine without a GPU and export every
+entry point that include/cgfd3d.h and include/cgfd_fno.h declare (no compute
+calls here)."""
 import ctypes
 import os
 import re
@@ -11,8 +12,8 @@ HEADER = os.path.join(ROOT, "include", "cgfd3d.h")
 LIB = os.path.join(ROOT, "copara-cgfd3d_amd", "libcgfd3d.so")
 
 
-def declared_symbols():
-    text = open(HEADER).read()
+def declared_symbols(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(cgfd_[a-z_0-9]+)\s*\(", text)))
 
@@ -57,3 +58,23 @@ def test_create_rejects_bad_dims_without_touching_the_gpu():
     mid = cgfd3d.Dims(900, 900, 900, 100.0, 0.01, 10)
     assert lib.cgfd_create(ctypes.byref(mid), 1, 0, ctypes.byref(h)) == -1
     assert not h.value
+
+
+FNO_HEADER = os.path.join(ROOT, "include", "cgfd_fno.h")
+FNO_LIB = os.path.join(ROOT, "copara-cgfd3d_amd", "libcgfd_fno.so")
+
+
+@pytest.mark.skipif(not os.path.exists(FNO_LIB), reason="libcgfd_fno.so not built")
+def test_fno_library_exports_and_validates_without_the_gpu():
+    names = declared_symbols(FNO_HEADER)
+    assert names == ["cgfd_fno_gemm_bf16"]
+    lib = ctypes.CDLL(FNO_LIB)
+    assert all(hasattr(lib, n) for n in names)
+    f = lib.cgfd_fno_gemm_bf16
+    L, P, I = ctypes.c_long, ctypes.c_void_p, ctypes.c_int
+    f.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P]
+    dummy = ctypes.c_void_p(16)
+    # rejected before any launch: empty sizes, null operands, too many row tiles
+    assert f(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 0, 4, 4, 1, 0, None) == -1
+    assert f(None, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 4, 1, 0, None) == -1
+    assert f(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 64 * 65536, 4, 4, 1, 0, None) == -1

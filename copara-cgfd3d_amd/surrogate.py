@@ -249,26 +249,71 @@ def draw_sources(n, base, seed):
     return np.array(out)
 
 
-def fd_scenario(base, p):
-    """The FD scenario dict of source parameters p."""
+def source_rows(params):
+    """(x, y, depth, mxx, myy, mzz, mxy, mxz, myz, log10 M0) rows: the unit
+    moment tensor of each (strike, dip, rake)."""
+    params = np.asarray(params, np.float64)
+    m = np.array([scenario.angle2moment(*p[3:6]) for p in params]).reshape(-1, 6)
+    return np.concatenate([params[:, :3], m, params[:, 6:7]], 1)
+
+
+# The hill medium is symmetric under the dihedral group of the square about
+# its centre (ni = nj, centred hill, flat layers, equal PML): each element is
+# (swap x and y, mirror x, mirror y).  A transformed source gives the
+# transformed PGV maps, up to the small asymmetry of the alternating one-sided
+# differences, so every FD run yields 8 training samples.
+D4 = [(t, fx, fy) for t in (0, 1) for fx in (0, 1) for fy in (0, 1)]
+
+
+def transform(base, src, pgv, g):
+    """Apply dihedral element g = (swap, mirror x, mirror y) to source rows
+    [n, 10] and, if given, PGV maps [n, 3, ni, nj]."""
+    t, fx, fy = g
+    ni, nj, dh = base["ni"], base["nj"], base["dh"]
+    if t and ni != nj:
+        raise ValueError("the x/y swap needs a square surface grid")
+    s = np.array(src, np.float64, copy=True)
+    P = None if pgv is None else np.array(pgv, copy=True)
+    if t:
+        s[:, [0, 1]] = s[:, [1, 0]]
+        s[:, 3:9] = s[:, [4, 3, 5, 6, 8, 7]]            # mxx<->myy, mxz<->myz
+        if P is not None:
+            P = P[:, [1, 0, 2]].transpose(0, 1, 3, 2)   # |Vx| <-> |Vy|
+    if fx:
+        s[:, 0] = (ni - 1) * dh - s[:, 0]
+        s[:, [6, 7]] *= -1                              # mxy, mxz
+        if P is not None:
+            P = P[:, :, ::-1, :]
+    if fy:
+        s[:, 1] = (nj - 1) * dh - s[:, 1]
+        s[:, [6, 8]] *= -1                              # mxy, myz
+        if P is not None:
+            P = P[:, :, :, ::-1]
+    return s, (None if P is None else np.ascontiguousarray(P))
+
+
+def fd_scenario(base, src):
+    """The FD scenario dict of one source row."""
     dh = base["dh"]
-    i, j, dk = int(round(p[0] / dh)), int(round(p[1] / dh)), int(round(p[2] / dh))
+    i, j, dk = int(round(src[0] / dh)), int(round(src[1] / dh)), int(round(src[2] / dh))
     sc = dict(base)
     sc["moment_locs"] = np.array([[i + 3, j + 3, base["nk"] - 1 - dk + 3]], np.int32)
-    sc["moment_vec"] = (scenario.angle2moment(p[3], p[4], p[5]) * 10 ** p[6]).astype(np.float32)[None, :]
+    sc["moment_vec"] = (np.asarray(src[3:9]) * 10 ** src[9]).astype(np.float32)[None, :]
     sc["moment_stf"] = base["stf"]
     return sc
 
 
-def run_fd(base, params, group=32, log=None):
-    """Surface PGV maps [n, 3, ni, nj] of the scenarios, advanced `group` at a
-    time with cgfd_run_batch; `log` gets a line per group."""
+def run_fd(base, params, group=32, log=None, rows=False):
+    """Surface PGV maps [n, 3, ni, nj] of the scenarios (parameter rows, or
+    source rows with rows=True), advanced `group` at a time with
+    cgfd_run_batch; `log` gets a line per group."""
     import cgfd3d
 
-    out = np.empty((len(params), 3, base["ni"], base["nj"]), np.float32)
-    for g0 in range(0, len(params), group):
+    src = np.asarray(params, np.float64) if rows else source_rows(params)
+    out = np.empty((len(src), 3, base["ni"], base["nj"]), np.float32)
+    for g0 in range(0, len(src), group):
         batch = []
-        for p in params[g0:g0 + group]:
+        for p in src[g0:g0 + group]:
             sc = fd_scenario(base, p)
             s = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
             s.setup(sc)
@@ -278,34 +323,40 @@ def run_fd(base, params, group=32, log=None):
             out[g0 + k] = s.get_pgv()[:, 3:-3, 3:-3]
             s.close()
         if log:
-            log(f"FD scenarios {g0 + len(batch)}/{len(params)}")
+            log(f"FD scenarios {g0 + len(batch)}/{len(src)}")
     return out
 
 
-def encode(base, params):
-    """Input fields [n, ni, nj, 12] on the surface grid: x and y, the source's
-    relative position and a 1 km bump around its epicentre, depth, hypocentral
-    distance, and the six unit moment-tensor components."""
+NCH = 13
+
+
+def encode(base, src):
+    """Input fields [n, ni, nj, 13] on the surface grid from source rows: x
+    and y, the source's relative position and a 1 km bump around its
+    epicentre, depth, hypocentral distance and its logarithm, and five of the
+    unit moment-tensor components (mzz = -(mxx + myy))."""
     ni, nj, dh = base["ni"], base["nj"], base["dh"]
     L = max(ni, nj) * dh
     X, Y = np.meshgrid(np.arange(ni) * dh, np.arange(nj) * dh, indexing="ij")
-    out = np.empty((len(params), ni, nj, 12), np.float32)
-    for n, p in enumerate(params):
+    out = np.empty((len(src), ni, nj, NCH), np.float32)
+    for n, p in enumerate(src):
         dx, dy = (X - p[0]) / L, (Y - p[1]) / L
         dz = p[2] / L
-        m = scenario.angle2moment(p[3], p[4], p[5])
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
         out[n, ..., 0], out[n, ..., 1] = X / L, Y / L
         out[n, ..., 2], out[n, ..., 3] = dx, dy
         out[n, ..., 4] = np.exp(-((X - p[0]) ** 2 + (Y - p[1]) ** 2) / (2 * 1000.0 ** 2))
         out[n, ..., 5] = dz
-        out[n, ..., 6] = np.sqrt(dx * dx + dy * dy + dz * dz)
-        out[n, ..., 7:12] = m[[0, 1, 3, 4, 5]]   # mzz = -(mxx + myy) adds nothing
+        out[n, ..., 6] = r
+        out[n, ..., 7] = np.log10(r)
+        out[n, ..., 8:13] = p[[3, 4, 6, 7, 8]]
     return out
 
 
-def target(pgv, params):
-    """log10(PGV / M0) [n, ni, nj, 3] (a floor keeps silent points finite)."""
-    m0 = 10.0 ** params[:, 6]
+def target(pgv, src):
+    """log10(PGV / M0) [n, ni, nj, 3] of source rows (a floor keeps silent
+    points finite)."""
+    m0 = 10.0 ** np.asarray(src)[:, 9]
     return np.log10(np.maximum(pgv / m0[:, None, None, None], 1e-30)).transpose(0, 2, 3, 1).astype(np.float32)
 
 
@@ -328,14 +379,21 @@ class Surrogate:
 
     def __init__(self, base, width=64, modes=16, layers=4, seeds=(0, 1), device="cuda"):
         self.base = base
-        self.nets = [FNO2d(12, 3, width, modes, layers, seed=s).to(device) for s in seeds]
+        self.nets = [FNO2d(NCH, 3, width, modes, layers, seed=s).to(device) for s in seeds]
         self.device = device
         self.mu = self.sd = None
         self.threshold = None
 
-    def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None):
-        x = torch.from_numpy(encode(self.base, params)).to(self.device)
-        y = target(pgv, params)
+    def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None, augment=False):
+        """Train both networks on parameter rows and their FD PGV maps; with
+        augment=True every run is used in all 8 symmetric positions."""
+        src = source_rows(params)
+        if augment:
+            pairs = [transform(self.base, src, pgv, g) for g in D4]
+            src = np.concatenate([p[0] for p in pairs])
+            pgv = np.concatenate([p[1] for p in pairs])
+        x = torch.from_numpy(encode(self.base, src)).to(self.device)
+        y = target(pgv, src)
         self.mu, self.sd = float(y.mean()), float(y.std())
         y = torch.from_numpy((y - self.mu) / self.sd).to(self.device)
         losses = []
@@ -362,7 +420,8 @@ class Surrogate:
     @torch.no_grad()
     def predict(self, params, batch=32):
         """PGV maps [n, 3, ni, nj] (the ensemble mean) and the indicator [n]."""
-        x = torch.from_numpy(encode(self.base, params)).to(self.device)
+        src = source_rows(params)
+        x = torch.from_numpy(encode(self.base, src)).to(self.device)
         outs = []
         for net in self.nets:
             outs.append(torch.cat([net(x[b:b + batch]) for b in range(0, len(x), batch)]))

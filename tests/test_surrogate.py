@@ -24,13 +24,41 @@ def _bf16(a):
 
 def test_encoding_and_target_shapes():
     base = {"ni": 20, "nj": 16, "dh": 100.0}
-    p = np.array([[500.0, 600.0, 1500.0, 30.0, 60.0, 90.0, 15.0]])
-    x = surrogate.encode(base, p)
-    assert x.shape == (1, 20, 16, 12)
+    src = surrogate.source_rows(np.array([[500.0, 600.0, 1500.0, 30.0, 60.0, 90.0, 15.0]]))
+    x = surrogate.encode(base, src)
+    assert x.shape == (1, 20, 16, surrogate.NCH)
     assert x[0, 5, 6, 4] == pytest.approx(1.0)          # the epicentral bump peaks at the source
     pgv = np.full((1, 3, 20, 16), 1e-3, np.float32)
-    t = surrogate.target(pgv, p)
+    t = surrogate.target(pgv, src)
     np.testing.assert_allclose(t, -3.0 - 15.0, atol=1e-5)   # log10(PGV / M0)
+
+
+def test_dihedral_transforms_are_orthogonal_similarities():
+    # each element maps the moment tensor to R M R^T: eigenvalues are kept;
+    # positions stay on the grid; the 8 images are distinct, and swapping or
+    # mirroring twice is the identity
+    base = {"ni": 40, "nj": 40, "dh": 100.0}
+    rng = np.random.default_rng(0)
+    p = np.array([[1200.0, 1700.0, 2000.0, *rng.uniform([0, 10, -180], [360, 90, 180]), 15.0]])
+    src = surrogate.source_rows(p)
+    pgv = rng.random((1, 3, 40, 40)).astype(np.float32)
+
+    def mat(r):
+        xx, yy, zz, xy, xz, yz = r[3:9]
+        return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]])
+
+    ev = np.linalg.eigvalsh(mat(src[0]))
+    images = set()
+    for g in surrogate.D4:
+        s2, p2 = surrogate.transform(base, src, pgv, g)
+        np.testing.assert_allclose(np.linalg.eigvalsh(mat(s2[0])), ev, atol=1e-12)
+        assert p2.shape == pgv.shape and np.isclose(np.sort(p2.ravel()), np.sort(pgv.ravel())).all()
+        images.add((round(s2[0, 0]), round(s2[0, 1])) + tuple(np.round(s2[0, 3:9], 9)))
+        if g in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            s3, p3 = surrogate.transform(base, s2, p2, g)
+            np.testing.assert_allclose(s3, src)
+            np.testing.assert_array_equal(p3, pgv)
+    assert len(images) == 8
 
 
 def test_library_missing_raises(monkeypatch):
@@ -109,7 +137,7 @@ def test_trained_network_beats_baseline_and_gate_falls_back_to_fd():
     losses = s.fit(ptrain, ytrain, epochs=60, batch=8, lr=3e-3)
     assert losses[59] < 0.3 * losses[0]
     pred, ind = s.predict(ptest)
-    t = surrogate.target(ytrain, ptrain).mean(0)                 # mechanism-blind baseline
+    t = surrogate.target(ytrain, surrogate.source_rows(ptrain)).mean(0)   # mechanism-blind baseline
     base_pred = np.stack([10.0 ** t.transpose(2, 0, 1) * 10.0 ** p[6] for p in ptest])
     for metric in (surrogate.pgv_error, surrogate.pgv_pointwise_error):
         e, b = np.median(metric(pred, ytest)), np.median(metric(base_pred, ytest))
@@ -120,3 +148,18 @@ def test_trained_network_beats_baseline_and_gate_falls_back_to_fd():
     got, ok = s.answer(ptest[:3], fd_group=3)
     assert not ok.any()
     np.testing.assert_array_equal(got, ytest[:3])
+
+
+@pytest.mark.gpu
+def test_fd_engine_is_symmetric_under_the_transforms():
+    # the FD answer for a transformed source is the transformed answer, up
+    # to the asymmetry of the alternating one-sided differences
+    base = surrogate.base_medium(ni=40, nj=40, nk=32, nt=200, dt=0.006, npml=8)
+    src = surrogate.source_rows(surrogate.draw_sources(1, base, seed=5))
+    pairs = [surrogate.transform(base, src, None, g)[0] for g in ((1, 0, 0), (0, 1, 1), (1, 1, 0))]
+    ys = surrogate.run_fd(base, np.concatenate([src] + pairs), group=4, rows=True)
+    for k, g in enumerate(((1, 0, 0), (0, 1, 1), (1, 1, 0))):
+        _, want = surrogate.transform(base, src, ys[:1], g)
+        err = float(surrogate.pgv_error(ys[k + 1:k + 2], want)[0])
+        print(f"symmetry {g}: {err:.4f}", flush=True)
+        assert err < 0.03, (g, err)

@@ -151,6 +151,12 @@ def surrogate_bench(args):
     err_sur = surrogate.pgv_error(pred, yte)
     err_pw = surrogate.pgv_pointwise_error(pred, yte)
     err_gate = surrogate.pgv_error(ans, yte)
+    rank = lambda v: np.argsort(np.argsort(v))
+    spearman = float(np.corrcoef(rank(ind), rank(err_sur))[0, 1])
+    s.indicator = "log"
+    _, ind_log = s.predict(pte)
+    s.indicator = "l2"
+    spearman_log = float(np.corrcoef(rank(ind_log), rank(err_sur))[0, 1])
     value = len(pte) / t_gate
     seq_rate = nseq / t_seq
     line = {
@@ -171,6 +177,9 @@ def surrogate_bench(args):
                    "median_pgv_error_surrogate_all": round(float(np.median(err_sur)), 4),
                    "median_pointwise_pgv_error_surrogate_all": round(float(np.median(err_pw)), 4),
                    "median_pgv_error_gated": round(float(np.median(err_gate)), 4),
+                   "indicator_error_rank_correlation": round(spearman, 3),
+                   "log_indicator_error_rank_correlation": round(spearman_log, 3),
+                   "best_decile_pgv_error": round(float(np.quantile(err_sur, 0.1)), 4),
                    "pgv_error": "relative L2 of the three surface PGV maps per scenario vs full FD"},
         "roofline": None, "cpu_baseline": None,
     }
@@ -262,7 +271,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=60, help="training epochs of the surrogate workload")
     ap.add_argument("--fno-width", type=int, default=48, help="FNO channel width of the surrogate workload")
     ap.add_argument("--fno-modes", type=int, default=12, help="retained Fourier modes per axis")
-    ap.add_argument("--augment", action="store_true", help="train on the 8 symmetric images of every FD run")
+    ap.add_argument("--augment", action="store_true", help="also train on the symmetric images of every FD run")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)

@@ -259,10 +259,13 @@ def source_rows(params):
 
 # The hill medium is symmetric under the dihedral group of the square about
 # its centre (ni = nj, centred hill, flat layers, equal PML): each element is
-# (swap x and y, mirror x, mirror y).  A transformed source gives the
-# transformed PGV maps, up to the small asymmetry of the alternating one-sided
-# differences, so every FD run yields 8 training samples.
+# (swap x and y, mirror x, mirror y).  The discrete scheme keeps a subgroup:
+# x and y share one forward/backward sign per stage, so swapping them is exact
+# and mirroring both stays within ~0.1 %, while mirroring one axis alone
+# changes the PGV maps by a few percent.  Training uses SYM, so every FD run
+# yields 4 samples.
 D4 = [(t, fx, fy) for t in (0, 1) for fx in (0, 1) for fy in (0, 1)]
+SYM = [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)]
 
 
 def transform(base, src, pgv, g):
@@ -383,13 +386,16 @@ class Surrogate:
         self.device = device
         self.mu = self.sd = None
         self.threshold = None
+        # error indicator: "l2" = relative L2 distance of the two networks'
+        # PGV maps; "log" = median |difference| of their log10 PGV
+        self.indicator = "l2"
 
     def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None, augment=False):
         """Train both networks on parameter rows and their FD PGV maps; with
-        augment=True every run is used in all 8 symmetric positions."""
+        augment=True every run is also used in its 3 images under SYM."""
         src = source_rows(params)
         if augment:
-            pairs = [transform(self.base, src, pgv, g) for g in D4]
+            pairs = [transform(self.base, src, pgv, g) for g in SYM]
             src = np.concatenate([p[0] for p in pairs])
             pgv = np.concatenate([p[1] for p in pairs])
         x = torch.from_numpy(encode(self.base, src)).to(self.device)
@@ -428,8 +434,11 @@ class Surrogate:
         m0 = torch.from_numpy(10.0 ** params[:, 6]).to(self.device).float()
         pgv = [10.0 ** (o * self.sd + self.mu) * m0[:, None, None, None] for o in outs]
         mean = sum(pgv) / len(pgv)
-        d = (pgv[0] - pgv[1]).reshape(len(x), -1)
-        ind = torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(mean.reshape(len(x), -1), dim=1)
+        if self.indicator == "log":
+            ind = torch.median((outs[0] - outs[1]).abs().reshape(len(x), -1), dim=1).values * self.sd
+        else:
+            d = (pgv[0] - pgv[1]).reshape(len(x), -1)
+            ind = torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(mean.reshape(len(x), -1), dim=1)
         return mean.permute(0, 3, 1, 2).cpu().numpy(), ind.cpu().numpy()
 
     def calibrate(self, params, pgv, target_err=0.05):

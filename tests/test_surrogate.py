@@ -152,14 +152,20 @@ def test_trained_network_beats_baseline_and_gate_falls_back_to_fd():
 
 @pytest.mark.gpu
 def test_fd_engine_is_symmetric_under_the_transforms():
-    # the FD answer for a transformed source is the transformed answer, up
-    # to the asymmetry of the alternating one-sided differences
+    # the FD answer for a transformed source is the transformed answer: the
+    # x/y swap exactly (x and y share the forward/backward sign of each
+    # stage), the mirror of both axes to 0.5 %; the training subgroup SYM is
+    # built from these
     base = surrogate.base_medium(ni=40, nj=40, nk=32, nt=200, dt=0.006, npml=8)
     src = surrogate.source_rows(surrogate.draw_sources(1, base, seed=5))
-    pairs = [surrogate.transform(base, src, None, g)[0] for g in ((1, 0, 0), (0, 1, 1), (1, 1, 0))]
-    ys = surrogate.run_fd(base, np.concatenate([src] + pairs), group=4, rows=True)
-    for k, g in enumerate(((1, 0, 0), (0, 1, 1), (1, 1, 0))):
+    gs = [g for g in surrogate.SYM if g != (0, 0, 0)] + [(1, 1, 0)]
+    pairs = [surrogate.transform(base, src, None, g)[0] for g in gs]
+    ys = surrogate.run_fd(base, np.concatenate([src] + pairs), group=len(gs) + 1, rows=True)
+    for k, g in enumerate(gs):
         _, want = surrogate.transform(base, src, ys[:1], g)
         err = float(surrogate.pgv_error(ys[k + 1:k + 2], want)[0])
         print(f"symmetry {g}: {err:.4f}", flush=True)
-        assert err < 0.03, (g, err)
+        if g == (1, 0, 0):
+            assert err < 1e-6
+        elif g in surrogate.SYM:
+            assert err < 5e-3, (g, err)

@@ -143,8 +143,8 @@ def surrogate_bench(args):
     ycal = surrogate.run_fd(base, pcal, group=32)
     yte = surrogate.run_fd(base, pte, group=32)
     s = surrogate.Surrogate(base, width=args.fno_width, modes=args.fno_modes, layers=4)
-    losses, t_train = surrogate.timed(lambda: s.fit(ptr, ytr, epochs=args.epochs, batch=16, lr=3e-3, log=log,
-                                                    augment=args.augment))
+    losses, t_train = surrogate.timed(lambda: s.fit(ptr, ytr, epochs=args.epochs, batch=args.sur_batch, lr=3e-3,
+                                                    log=log, augment=args.augment))
     thr = s.calibrate(pcal, ycal, target_err=0.05)
     (pred, ind), t_sur = surrogate.timed(lambda: s.predict(pte))
     (ans, ok), t_gate = surrogate.timed(lambda: s.answer(pte))
@@ -166,7 +166,7 @@ def surrogate_bench(args):
         "config": {"workload": "surrogate-hill-80x80x48-nt600", "train_scenarios": len(ptr),
                    "calibration_scenarios": len(pcal), "test_scenarios": len(pte), "epochs": args.epochs,
                    "fno": {"width": args.fno_width, "modes": args.fno_modes, "layers": 4, "ensemble": 2},
-                   "symmetry_augmentation": bool(args.augment),
+                   "symmetry_augmentation": bool(args.augment), "train_batch": args.sur_batch,
                    "fd_sequential_scenarios_per_s": round(seq_rate, 3),
                    "fd_batched_scenarios_per_s": round(len(ptr) / t_fd, 3),
                    "surrogate_only_scenarios_per_s": round(len(pte) / t_sur, 1),
@@ -272,6 +272,7 @@ def main():
     ap.add_argument("--fno-width", type=int, default=48, help="FNO channel width of the surrogate workload")
     ap.add_argument("--fno-modes", type=int, default=12, help="retained Fourier modes per axis")
     ap.add_argument("--augment", action="store_true", help="also train on the symmetric images of every FD run")
+    ap.add_argument("--sur-batch", type=int, default=16, help="training batch size of the surrogate workload")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)

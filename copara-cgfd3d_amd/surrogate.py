@@ -389,6 +389,8 @@ class Surrogate:
         # error indicator: "l2" = relative L2 distance of the two networks'
         # PGV maps; "log" = median |difference| of their log10 PGV
         self.indicator = "l2"
+        # test-time averaging over the symmetric images of each query
+        self.tta = False
 
     def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None, augment=False):
         """Train both networks on parameter rows and their FD PGV maps; with
@@ -424,14 +426,31 @@ class Surrogate:
         return losses
 
     @torch.no_grad()
-    def predict(self, params, batch=32):
-        """PGV maps [n, 3, ni, nj] (the ensemble mean) and the indicator [n]."""
-        src = source_rows(params)
-        x = torch.from_numpy(encode(self.base, src)).to(self.device)
-        outs = []
-        for net in self.nets:
-            outs.append(torch.cat([net(x[b:b + batch]) for b in range(0, len(x), batch)]))
-        m0 = torch.from_numpy(10.0 ** params[:, 6]).to(self.device).float()
+    def _log_maps(self, src, batch):
+        """Each network's normalised log10(PGV/M0) [n, ni, nj, 3] for source
+        rows; with self.tta, averaged over the SYM images of each source
+        (mapped back: every element of SYM is its own inverse)."""
+        images = SYM if self.tta else [(0, 0, 0)]
+        outs = [0.0 for _ in self.nets]
+        for g in images:
+            sg, _ = transform(self.base, src, None, g)
+            x = torch.from_numpy(encode(self.base, sg)).to(self.device)
+            for k, net in enumerate(self.nets):
+                o = torch.cat([net(x[b:b + batch]) for b in range(0, len(x), batch)])
+                if g != (0, 0, 0):
+                    o = o.permute(0, 3, 1, 2).cpu().numpy()
+                    o = torch.from_numpy(transform(self.base, sg, o, g)[1]).to(self.device).permute(0, 2, 3, 1)
+                outs[k] = outs[k] + o / len(images)
+        return outs
+
+    @torch.no_grad()
+    def predict(self, params, batch=32, rows=False):
+        """PGV maps [n, 3, ni, nj] (the ensemble mean) and the indicator [n]
+        for parameter rows (or source rows with rows=True)."""
+        src = np.asarray(params, np.float64) if rows else source_rows(params)
+        outs = self._log_maps(src, batch)
+        x = outs[0]
+        m0 = torch.from_numpy(10.0 ** src[:, 9]).to(self.device).float()
         pgv = [10.0 ** (o * self.sd + self.mu) * m0[:, None, None, None] for o in outs]
         mean = sum(pgv) / len(pgv)
         if self.indicator == "log":

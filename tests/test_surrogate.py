@@ -169,3 +169,18 @@ def test_fd_engine_is_symmetric_under_the_transforms():
             assert err < 1e-6
         elif g in surrogate.SYM:
             assert err < 5e-3, (g, err)
+
+
+@pytest.mark.gpu
+def test_symmetry_averaged_prediction_is_equivariant():
+    # averaging over the images of SYM makes the predictor commute with SYM:
+    # the prediction for a transformed source is the transformed prediction
+    base = surrogate.base_medium(ni=40, nj=40, nk=32, nt=100, dt=0.006, npml=8)
+    s = surrogate.Surrogate(base, width=16, modes=6, layers=2)
+    s.mu, s.sd, s.tta = -20.0, 1.0, True
+    src = surrogate.source_rows(surrogate.draw_sources(3, base, seed=9))
+    pred, _ = s.predict(src, rows=True)
+    for g in surrogate.SYM[1:]:
+        sg, want = surrogate.transform(base, src, pred, g)
+        got, _ = s.predict(sg, rows=True)
+        assert np.abs(got / want - 1).max() < 1e-4, g

@@ -198,6 +198,33 @@ def surrogate_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def fp64_error(sc, steps):
+    """The FP32 run against the FP64 build of the same kernels on the same
+    inputs: relative L2 of each of the 9 wavefield components after `steps`
+    steps (largest and mean)."""
+    import numpy as np
+
+    import cgfd3d
+
+    fields = {}
+    for prec in (cgfd3d.FP32, cgfd3d.FP64):
+        g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], precision=prec)
+        if prec == cgfd3d.FP64:
+            sc = dict(sc, **{k: sc[k].astype(np.float64) for k in ("alpha", "beta", "rho")})
+        g.setup(sc)
+        g.run(steps)
+        g.sync()
+        fields[prec] = [g.get_field(c) for c in range(9)]
+        g.close()
+    errs = []
+    for a, b in zip(fields[cgfd3d.FP32], fields[cgfd3d.FP64]):
+        d = np.linalg.norm((a.astype(np.float64) - b).ravel())
+        n = np.linalg.norm(b.ravel())
+        errs.append(d / n if n > 0 else 0.0)
+    return {"steps": steps, "rel_l2_max": float(f"{max(errs):.3e}"), "rel_l2_mean": float(f"{np.mean(errs):.3e}"),
+            "what": f"FP32 vs FP64 GPU build, the 9 wavefield components after {steps} steps of the same problem"}
+
+
 def cpu_baseline(sc, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -286,6 +313,7 @@ def main():
     ap.add_argument("--augment", action="store_true", help="also train on the symmetric images of every FD run")
     ap.add_argument("--sur-batch", type=int, default=16, help="training batch size of the surrogate workload")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--fp64-steps", type=int, default=20, help="steps of the FP64-build error check (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0,
@@ -346,6 +374,12 @@ def main():
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)
 
+    fp64 = None
+    if args.fp64_steps > 0:
+        log(f"FP64 reference run ({args.fp64_steps} steps)")
+        fp64 = fp64_error(sc, args.fp64_steps)
+        log(f"FP32 vs FP64: max relative L2 {fp64['rel_l2_max']:.2e}")
+
     cpu = None
     if args.cpu_steps > 0:
         log("timing the CPU baseline")
@@ -354,7 +388,7 @@ def main():
 
     cfg = describe(sc)
     cfg.update({"stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
-                "bytes_per_stage_model": bytes_stage, "bytes_per_stage_loaded": bytes_loaded,
+                "bytes_per_stage_model": bytes_stage, "bytes_per_stage_loaded": bytes_loaded, "fp64_oracle": fp64,
                 "hbm_frac_of_loaded_bytes": round(bytes_loaded / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "finite_output": finite})
     line = {

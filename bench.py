@@ -156,6 +156,7 @@ def surrogate_bench(args):
         res[tta] = {"threshold": thr, "t_sur": t_sur, "t_gate": t_gate, "ok": ok, "err": err,
                     "err_pw": surrogate.pgv_pointwise_error(pred, yte),
                     "err_gate": surrogate.pgv_error(ans, yte),
+                    "err_acc": err[ok],
                     "rho": float(np.corrcoef(rank(ind), rank(err))[0, 1])}
         log(f"tta={tta}: median PGV error {np.median(err):.4f}, accepted {ok.mean():.3f}")
     s.tta = False
@@ -185,6 +186,8 @@ def surrogate_bench(args):
                    "median_pgv_error_surrogate_all": round(float(np.median(r["err"])), 4),
                    "median_pointwise_pgv_error_surrogate_all": round(float(np.median(r["err_pw"])), 4),
                    "median_pgv_error_gated": round(float(np.median(r["err_gate"])), 4),
+                   "median_pgv_error_accepted": (round(float(np.median(r["err_acc"])), 4) if len(r["err_acc"])
+                                                 else None),
                    "indicator_error_rank_correlation": round(r["rho"], 3),
                    "best_decile_pgv_error": round(float(np.quantile(r["err"], 0.1)), 4),
                    "without_averaging": {"median_pgv_error": round(float(np.median(res[False]["err"])), 4),

@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "copara-cgfd3d_amd"))
 
 HBM_PEAK_GBS = 8000.0
+PROFILE_STEPS = 8
 
 
 def log(msg):
@@ -80,6 +81,7 @@ def batch_bench(args):
     for sc in scs:
         g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
         g.setup(sc)
+        g.set_graph(args.graph)
         sims.append(g)
     zch = [g.autotune(trial_steps=2) for g in sims]
     cells = sum(sc["ni"] * sc["nj"] * sc["nk"] for sc in scs)
@@ -110,7 +112,8 @@ def batch_bench(args):
         "metric": "Gcell-RKstages/s", "value": round(value, 4), "unit": "Gcell-RKstages/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_bat / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"batch-{len(scs)}-hill", "scenarios": [sc["name"] for sc in scs],
+        "config": {"workload": f"batch-{len(scs)}-hill", "graph_replay": bool(args.graph),
+                   "scenarios": [sc["name"] for sc in scs],
                    "zchunks": zch, "cells_total": cells,
                    "scenario_steps_per_s_batched": round(len(scs) * args.steps / t_bat, 2),
                    "scenario_steps_per_s_sequential": round(len(scs) * args.steps / t_seq, 2),
@@ -319,6 +322,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20, help="steps of the FP64-build error check (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--graph", action="store_true", help="replay captured 4-step cycles (cgfd_set_graph)")
     ap.add_argument("--variant", type=int, default=0,
                     help="stage kernel: 0 = LDS-staged planes, 1 = direct loads, 2 = split velocity/stress")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -338,12 +342,14 @@ def main():
 
     import cgfd3d
 
-    nt = args.warmup + args.steps
+    nt = args.warmup + args.steps + (PROFILE_STEPS if args.graph else 0)
     t0 = time.perf_counter()
     sc = build_workload(args.workload, nt)
     log(f"workload {sc['name']} built in {time.perf_counter() - t0:.1f} s")
     sim = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
     sim.set_variant(args.variant)
+    if args.graph:
+        sim.set_graph(True)
     if args.zchunks or args.rows:
         sim.set_tuning(args.zchunks, args.rows)
     t0 = time.perf_counter()
@@ -354,12 +360,19 @@ def main():
     sim.run(args.warmup)
     sim.sync()
     log(f"{args.warmup} warmup steps done")
-    sim.profile(True)
+    # kernel durations come from HIP events around each launch; graph replay
+    # is inactive while they are recorded, so a graph run is timed without
+    # them and the kernel durations come from PROFILE_STEPS eager steps after it
+    sim.profile(not args.graph)
     sim.sync()
     t0 = time.perf_counter()
     sim.run(args.steps)
     sim.sync()
     t = time.perf_counter() - t0
+    if args.graph:
+        sim.profile(True)
+        sim.run(PROFILE_STEPS)
+        sim.sync()
     prof = sim.profile_read()
     sim.profile(False)
     log(f"{args.steps} timed steps in {t:.3f} s")
@@ -390,7 +403,7 @@ def main():
         log(f"CPU baseline {cpu['value']:.4f} Gcell-RKstages/s on {cpu['cores']} threads")
 
     cfg = describe(sc)
-    cfg.update({"stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
+    cfg.update({"graph_replay": bool(args.graph), "stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
                 "bytes_per_stage_model": bytes_stage, "bytes_per_stage_loaded": bytes_loaded, "fp64_oracle": fp64,
                 "hbm_frac_of_loaded_bytes": round(bytes_loaded / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "finite_output": finite})

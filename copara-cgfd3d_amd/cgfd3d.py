@@ -53,6 +53,7 @@ SIGNATURES = {
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_sync": (ctypes.c_int, [_VP]),
+    "cgfd_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_batch": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
     "cgfd_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
     "cgfd_current_step": (ctypes.c_int, [_VP]),
@@ -306,6 +307,10 @@ class Simulation:
 
     def run_stages(self, nstages):
         self._check(self.lib.cgfd_run_stages(self.h, int(nstages)), "run_stages")
+
+    def set_graph(self, on=True):
+        """Replay captured 4-step cycles in run() (small, launch-bound grids)."""
+        self._check(self.lib.cgfd_set_graph(self.h, int(bool(on))), "set_graph")
 
     def sync(self):
         self._check(self.lib.cgfd_sync(self.h), "sync")

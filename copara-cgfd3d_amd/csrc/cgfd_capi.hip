@@ -88,7 +88,11 @@ struct cgfd_ctx {
     std::vector<Snapshot> snaps;
 
     int currT = 0, stage = 0;
+    int *d_step = nullptr; // device copy of currT, read by the stage, trace and snapshot kernels
     bool inited = false;
+    // cgfd_run replays captured 4-step cycles, one graph per starting phase currT % 4
+    bool use_graph = false, capturing = false;
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     int rows = 4, zchunks = 1;
     int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
     char *args_d = nullptr; // the 4 per-stage argument blocks in device memory
@@ -239,7 +243,8 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
         (rc = dalloc(c, &c->staging, (size_t)c->nref * c->esz)) ||
         (rc = dalloc(c, &c->timg, (size_t)9 * c->tstr * c->esz)) ||
         (rc = dalloc(c, &c->cov, (size_t)18 * c->cstr * c->esz)) ||
-        (rc = dalloc(c, &c->pgv, (size_t)3 * c->cstr * c->esz))) {
+        (rc = dalloc(c, &c->pgv, (size_t)3 * c->cstr * c->esz)) ||
+        (rc = dalloc(c, (char **)&c->d_step, sizeof(int)))) {
         fprintf(stderr, "cgfd_create: %s\n", c->err.c_str());
         cgfd_destroy(c);
         return rc;
@@ -285,6 +290,14 @@ static void free_sources(cgfd_ctx *c)
     c->nent = 0;
 }
 
+static void drop_graphs(cgfd_ctx *c)
+{
+    for (auto &g : c->gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 static void drop_events(cgfd_ctx *c)
 {
     for (auto &p : c->ev_stage) {
@@ -305,6 +318,8 @@ void cgfd_destroy(cgfd_ctx *c)
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_events(c);
+    drop_graphs(c);
+    dfree_t(c->d_step);
     dfree(c->sets);
     dfree(c->met);
     dfree(c->med);
@@ -738,6 +753,8 @@ int cgfd_init(cgfd_ctx *c)
         if (c->aux[a]) HIPCHK(c, hipMemsetAsync(c->aux[a], 0, (size_t)NSETS * NF * c->astr[a] * c->esz, c->stream));
     HIPCHK(c, hipMemsetAsync(c->pgv, 0, (size_t)3 * c->cstr * c->esz, c->stream));
     HIPCHK(c, hipMemsetAsync(c->timg, 0, (size_t)9 * c->tstr * c->esz, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_step, 0, sizeof(int), c->stream));
+    drop_graphs(c);
     if (c->seis) HIPCHK(c, hipMemsetAsync(c->seis, 0, (size_t)NF * c->nrecv * c->d.nt * c->esz, c->stream));
     for (auto &sn : c->snaps)
         HIPCHK(c, hipMemsetAsync(sn.buf, 0, (size_t)sn.ntime * sn.ncomp * sn.npts * c->esz, c->stream));
@@ -829,6 +846,7 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     sa.beta = RK4B[k] * dt;
     sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
     sa.hmet = (c->hmet && c->use_hmet) ? 1 : 0;
+    sa.step = c->d_step;
     return sa;
 }
 
@@ -853,6 +871,7 @@ static int upload_args(cgfd_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->args_d, host.data(), host.size(), hipMemcpyHostToDevice));
     c->args_dirty = false;
+    drop_graphs(c); // captured launches refer to the old blocks' contents
     return 0;
 }
 
@@ -861,8 +880,7 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     const int k = c->stage;
     int sx, sz;
     fb_signs(c->currT, k, &sx, &sz);
-    const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
-    const int tindex = 2 * c->currT + tinc; // calculate.cu:1404-1421
+    const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2); // Tindex = 2*currT + tinc, calculate.cu:1404-1421
     const int sin_ = stage_in(k);
     if (c->args_dirty) {
         int rc = upload_args(c);
@@ -898,7 +916,7 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
     }
-    launch_stage<R>(c->variant, sx, sz, k, sa, pa, tindex, c->rows, c->zchunks, c->stream);
+    launch_stage<R>(c->variant, sx, sz, k, sa, pa, tinc, c->rows, c->zchunks, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         c->ev_timg.push_back({ev[0], ev[1]});
@@ -908,14 +926,16 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     if (k == 3) {
         if (c->currT < c->d.nt && c->nrecv)
             launch_record<R>((const R *)c->set_ptr(SET_FW), c->g.fs, c->rlin, c->nrecv, (R *)c->seis, c->d.nt,
-                             c->currT, c->stream);
-        // SWpick after the step (simulation.cc:312-315): every tinv steps
+                             c->d_step, c->stream);
+        // SWpick after the step (simulation.cc:312-315): every tinv steps.  The
+        // kernel checks the step itself; a captured cycle launches it every step.
         for (auto &sn : c->snaps) {
             const int tinv = sn.spec[9];
-            if (c->currT % tinv != 0 || c->currT / tinv >= sn.ntime) continue;
-            R *slot = (R *)sn.buf + (size_t)(c->currT / tinv) * sn.ncomp * sn.npts;
-            launch_snap<R>((const R *)c->set_ptr(SET_FW), c->g.fs, sn.lin, sn.npts, sn.c0, sn.ncomp, slot, c->stream);
+            if (!c->capturing && (c->currT % tinv != 0 || c->currT / tinv >= sn.ntime)) continue;
+            launch_snap<R>((const R *)c->set_ptr(SET_FW), c->g.fs, sn.lin, sn.npts, sn.c0, sn.ncomp, (R *)sn.buf,
+                           c->d_step, tinv, sn.ntime, c->stream);
         }
+        launch_step_advance(c->d_step, c->stream);
         HIPCHK(c, hipGetLastError());
         c->currT++;
         c->stage = 0;
@@ -942,11 +962,82 @@ int cgfd_run_stages(cgfd_ctx *c, int nstages)
     return 0;
 }
 
+// Capture the 16 stages of the 4 steps from the current one (phase currT % 4
+// fixes their forward/backward pattern) into a graph.  Every step-dependent
+// value reaches the kernels through the device step counter, so the graph
+// replays at any step of the same phase.
+static int capture_cycle(cgfd_ctx *c, int phase)
+{
+    if (c->args_dirty) {
+        int rc = upload_args(c); // outside the capture: a copy must not become a graph node
+        if (rc) return rc;
+    }
+    const int t0 = c->currT;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    int rc = 0;
+    for (int s = 0; s < 16 && !rc; s++) rc = c->prec == CGFD_FP32 ? run_stage<float>(c) : run_stage<double>(c);
+    c->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->currT = t0;
+    c->stage = 0;
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : fail(c, CGFD_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&c->gexec[phase], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->gexec[phase] = nullptr;
+        return fail(c, CGFD_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// Replay (capturing first if needed) the 4-step cycle that starts at the
+// current step, on the context's stream.
+static int launch_cycle(cgfd_ctx *c)
+{
+    const int phase = c->currT % 4;
+    if (c->args_dirty) {
+        int rc = upload_args(c);
+        if (rc) return rc;
+    }
+    if (!c->gexec[phase]) {
+        int rc = capture_cycle(c, phase);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec[phase], c->stream));
+    c->currT += 4;
+    return 0;
+}
+
 int cgfd_run(cgfd_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return CGFD_EINVAL;
     if (c->stage != 0) return fail(c, CGFD_ESTATE, "cgfd_run called in the middle of a step");
+    if (c->use_graph && !c->prof && nsteps >= 4) {
+        if (!c->inited) return fail(c, CGFD_ESTATE, "cgfd_init must run after the last set_* call");
+        if ((long)c->currT + nsteps > c->d.nt)
+            return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
+        HIPCHK(c, hipSetDevice(c->dev));
+        for (; nsteps >= 4; nsteps -= 4) {
+            int rc = launch_cycle(c);
+            if (rc) return rc;
+        }
+    }
     return cgfd_run_stages(c, 4 * nsteps);
+}
+
+// Graph replay of whole 4-step cycles in cgfd_run (no reference
+// counterpart): for small grids whose steps are bound by launch gaps.
+int cgfd_set_graph(cgfd_ctx *c, int on)
+{
+    if (!c) return CGFD_EINVAL;
+    c->use_graph = on != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return 0;
 }
 
 // Many independent simulations advanced together: stage k of every context
@@ -963,6 +1054,19 @@ int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
         if (c->stage != 0) return fail(c, CGFD_ESTATE, "cgfd_run_batch called in the middle of a step");
         if ((long)c->currT + nsteps > c->d.nt)
             return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
+    }
+    bool graphs = n > 0;
+    for (int i = 0; i < n; i++) graphs = graphs && ctxs[i]->use_graph && !ctxs[i]->prof;
+    if (graphs) {
+        // whole 4-step cycles: each context's captured cycle on its own stream,
+        // the contexts' cycles enqueued in turn so the streams overlap
+        for (; nsteps >= 4; nsteps -= 4)
+            for (int i = 0; i < n; i++) {
+                cgfd_ctx *c = ctxs[i];
+                HIPCHK(c, hipSetDevice(c->dev));
+                int rc = launch_cycle(c);
+                if (rc) return rc;
+            }
     }
     for (int t = 0; t < nsteps; t++)
         for (int k = 0; k < 4; k++)

@@ -41,6 +41,7 @@ template <typename R> struct StageArgs {
     R alpha, beta;
     int zlen;
     int hmet; // 1: horizontal metrics are the constants 1/0 (variant 0 loads only the zeta metrics)
+    const int *step; // device step counter: the current step currT (Tindex = 2*currT + tinc)
 };
 
 template <typename R> struct TimgArgs {
@@ -60,9 +61,10 @@ template <typename R> struct TimgArgs {
 
 // variant 0: LDS-staged xy planes (default); 1: direct loads of the in-plane
 // taps.  `a` is the host copy of the argument block (launch geometry), `pa`
-// the same block in device memory, read by the kernel.
+// the same block in device memory, read by the kernel.  tinc = 0, 1, 1, 2 in
+// stages 0-3: the kernel forms Tindex = 2*(*a.step) + tinc.
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex,
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
                   int rows, int zchunks, hipStream_t s);
 template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s);
 template <typename R>
@@ -75,9 +77,15 @@ template <typename R>
 void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent, const R *fvec, const R *mvec,
                      R *scoef, int nent, R hr, hipStream_t s);
 template <typename R> void launch_media_prep(const Geom &g, R *med, hipStream_t s);
+// Trace and snapshot gathers after a step read the step number from the
+// device counter, so a captured step graph replays at any step.
 template <typename R>
-void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s);
+void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, const int *step,
+                   hipStream_t s);
 template <typename R>
-void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot, hipStream_t s);
+void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *buf, const int *step,
+                 int tinv, int ntime, hipStream_t s);
+// step counter += 1 (one thread)
+void launch_step_advance(int *step, hipStream_t s);
 
 } // namespace cgfd

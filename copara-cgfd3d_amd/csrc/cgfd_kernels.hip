@@ -492,10 +492,11 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
 // (vector L1/L2); one lane per (x,y) column sweeps its z chunk with the z taps
 // in a 5-deep register window.
 template <typename R, int SX, int SZ, int ST>
-__global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *pa, int tindex)
+__global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *pa, int tinc)
 {
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     const AP a = (AP)pa;
+    const int tindex = 2 * *a->step + tinc;
     const Geom g = geom_of(a);
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -582,13 +583,14 @@ template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
 template <typename R, int SX, int SZ, int ST, int HM>
-__global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tindex)
+__global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tinc)
 {
     // The argument block lives in device memory and is read through the
     // constant address space, so fields are fetched with scalar loads where
     // they are used instead of being preloaded into (and spilled from) SGPRs.
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     const AP a0 = (AP)pa;
+    const int tindex = 2 * *a0->step + tinc;
     constexpr int TY = lds_ty<R>(), HX = 64 + 4, HY = TY + 4;
     constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
     constexpr int ZOFF = SZ > 0 ? -1 : -3;
@@ -806,7 +808,7 @@ template <> struct SplitFields<1> { // stencil inputs: the three velocities
 template <typename R, int SX, int SZ, int ST, int PART>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (PART == 0 ? CGFD_SPLIT_WAVES_V : CGFD_SPLIT_WAVES_T) : 2, 8)))
-stage_split_kernel(const StageArgs<R> *pa, int tindex)
+stage_split_kernel(const StageArgs<R> *pa, int tinc)
 {
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     constexpr int NIN = SplitFields<PART>::n;
@@ -817,6 +819,7 @@ stage_split_kernel(const StageArgs<R> *pa, int tindex)
     __shared__ R lds[NIN][HY][HX];
 
     const AP a0 = (AP)pa;
+    const int tindex = 2 * *a0->step + tinc;
     const Geom g = geom_of(a0);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
     const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
@@ -1442,19 +1445,25 @@ __global__ void src_prep_kernel(Geom g, const R *met, const R *med, const int4 *
 
 // receivers (PWpick, calculate.cu:2780-2828): all 9 components of FW
 template <typename R>
-__global__ void record_kernel(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t)
+__global__ void record_kernel(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, const int *step)
 {
+    const int t = *step;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrecv * NF) return;
     const int r = i / NF, c = i % NF;
     seis[((long)c * nrecv + r) * nt + t] = fw[c * fs + rlin[r]];
 }
 
-// snapshots (SWpick, calculate.cu:2360-2444): ncomp components from c0 of FW
-// at npts points into one time slot laid out [comp][point]
+// snapshots (SWpick, calculate.cu:2360-2444): after step t with t % tinv == 0,
+// ncomp components from c0 of FW at npts points into time slot t/tinv, laid
+// out [comp][point]; other steps return at once
 template <typename R>
-__global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot)
+__global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *buf,
+                            const int *step, int tinv, int ntime)
 {
+    const int t = *step;
+    if (t % tinv != 0 || t / tinv >= ntime) return;
+    R *slot = buf + (long)(t / tinv) * ncomp * npts;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)npts * ncomp) return;
     const int c = (int)(i / npts), p = (int)(i % npts);
@@ -1465,46 +1474,46 @@ __global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, int SX, int SZ, int ST>
-static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex, int rows,
+static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc, int rows,
                              int zchunks, hipStream_t s)
 {
     if (variant == 0) {
         constexpr int ty = lds_ty<R>();
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
         if (a.hmet)
-            stage_lds_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
+            stage_lds_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tinc);
         else
-            stage_lds_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tindex);
+            stage_lds_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tinc);
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
-        stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tindex);
-        stage_split_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tindex);
+        stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tinc);
+        stage_split_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tinc);
     } else {
         dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, zchunks);
-        stage_direct_kernel<R, SX, SZ, ST><<<grid, dim3(64, rows), 0, s>>>(pa, tindex);
+        stage_direct_kernel<R, SX, SZ, ST><<<grid, dim3(64, rows), 0, s>>>(pa, tinc);
     }
 }
 
 template <typename R, int SX, int SZ>
-static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex, int rows,
+static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc, int rows,
                              int zchunks, hipStream_t s)
 {
     switch (st) {
-    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, tindex, rows, zchunks, s); break;
-    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, tindex, rows, zchunks, s); break;
-    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, tindex, rows, zchunks, s); break;
-    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, tindex, rows, zchunks, s); break;
+    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, tinc, rows, zchunks, s); break;
+    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, tinc, rows, zchunks, s); break;
+    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, tinc, rows, zchunks, s); break;
+    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, tinc, rows, zchunks, s); break;
     }
 }
 
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tindex,
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
                   int rows, int zchunks, hipStream_t s)
 {
-    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, tindex, rows, zchunks, s);
-    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, tindex, rows, zchunks, s);
-    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, tindex, rows, zchunks, s);
-    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, tindex, rows, zchunks, s);
+    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, tinc, rows, zchunks, s);
+    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, tinc, rows, zchunks, s);
+    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, tinc, rows, zchunks, s);
+    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, tinc, rows, zchunks, s);
 }
 
 template <typename R, int SPLIT>
@@ -1558,20 +1567,30 @@ void launch_src_prep(const Geom &g, const R *met, const R *med, const int4 *sent
 }
 
 template <typename R>
-void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *slot, hipStream_t s)
+void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int ncomp, R *buf, const int *step,
+                 int tinv, int ntime, hipStream_t s)
 {
     const long n = (long)npts * ncomp;
     if (n <= 0) return;
-    snap_kernel<R><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(fw, fs, lin, npts, c0, ncomp, slot);
+    snap_kernel<R><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(fw, fs, lin, npts, c0, ncomp, buf, step, tinv,
+                                                                ntime);
 }
 
 template <typename R>
-void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, int t, hipStream_t s)
+void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, int nt, const int *step,
+                   hipStream_t s)
 {
     if (nrecv <= 0) return;
     const int n = nrecv * NF;
-    record_kernel<R><<<(n + 255) / 256, 256, 0, s>>>(fw, fs, rlin, nrecv, seis, nt, t);
+    record_kernel<R><<<(n + 255) / 256, 256, 0, s>>>(fw, fs, rlin, nrecv, seis, nt, step);
 }
+
+__global__ void step_advance_kernel(int *step)
+{
+    if (threadIdx.x == 0) step[0] += 1;
+}
+
+void launch_step_advance(int *step, hipStream_t s) { step_advance_kernel<<<1, 64, 0, s>>>(step); }
 
 #define CGFD_INSTANTIATE(R)                                                                                   \
     template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int,  \
@@ -1582,8 +1601,9 @@ void launch_record(const R *fw, long fs, const long *rlin, int nrecv, R *seis, i
     template void launch_from_padded<R>(const R *, R *, const Geom &, hipStream_t);                          \
     template void launch_src_prep<R>(const Geom &, const R *, const R *, const int4 *, const R *, const R *,  \
                                      R *, int, R, hipStream_t);                                               \
-    template void launch_record<R>(const R *, long, const long *, int, R *, int, int, hipStream_t);       \
-    template void launch_snap<R>(const R *, long, const long *, int, int, int, R *, hipStream_t);          \
+    template void launch_record<R>(const R *, long, const long *, int, R *, int, const int *, hipStream_t); \
+    template void launch_snap<R>(const R *, long, const long *, int, int, int, R *, const int *, int, int,   \
+                                 hipStream_t);                                                              \
     template void launch_media_prep<R>(const Geom &, R *, hipStream_t);
 
 CGFD_INSTANTIATE(float)

@@ -306,10 +306,11 @@ def fd_scenario(base, src):
     return sc
 
 
-def run_fd(base, params, group=32, log=None, rows=False):
+def run_fd(base, params, group=32, log=None, rows=False, graph=True):
     """Surface PGV maps [n, 3, ni, nj] of the scenarios (parameter rows, or
     source rows with rows=True), advanced `group` at a time with
-    cgfd_run_batch; `log` gets a line per group."""
+    cgfd_run_batch (replaying captured step cycles when graph is set; the
+    results are the same); `log` gets a line per group."""
     import cgfd3d
 
     src = np.asarray(params, np.float64) if rows else source_rows(params)
@@ -320,6 +321,7 @@ def run_fd(base, params, group=32, log=None, rows=False):
             sc = fd_scenario(base, p)
             s = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
             s.setup(sc)
+            s.set_graph(graph)
             batch.append(s)
         cgfd3d.run_batch(batch, base["nt"])
         for k, s in enumerate(batch):

@@ -131,6 +131,14 @@ int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
  * z-chunk counts around the default, keep the fastest (*chosen), and
  * re-initialise the context.  Call right after cgfd_init. */
 int cgfd_autotune(cgfd_ctx *ctx, int trial_steps, int *chosen);
+/* Graph replay (no reference counterpart; for small grids whose steps are
+ * bound by the gaps between dependent launches): with on != 0, cgfd_run
+ * captures the launches of 4 whole steps once per starting phase (step % 4
+ * fixes the forward/backward pattern) into a HIP graph and replays it; the
+ * step number reaches the kernels through a device-side counter, so results
+ * equal eager stepping.  Steps left over (fewer than 4) run eagerly.
+ * Inactive while profiling. */
+int cgfd_set_graph(cgfd_ctx *ctx, int on);
 int cgfd_sync(cgfd_ctx *ctx);
 int cgfd_current_step(const cgfd_ctx *ctx);
 int cgfd_current_stage(const cgfd_ctx *ctx);

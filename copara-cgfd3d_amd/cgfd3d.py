@@ -300,7 +300,8 @@ class Simulation:
         self._check(self.lib.cgfd_run(self.h, int(nsteps)), "run")
 
     def autotune(self, trial_steps=3):
-        """Pick the fastest z-chunk count for this grid; re-initialises."""
+        """Pick the fastest z-chunk count for this grid (and, with trial_steps >= 4,
+        whether graph replay is faster); re-initialises."""
         z = ctypes.c_int(0)
         self._check(self.lib.cgfd_autotune(self.h, int(trial_steps), ctypes.byref(z)), "autotune")
         return z.value

@@ -1124,11 +1124,28 @@ int cgfd_autotune(cgfd_ctx *c, int trial_steps, int *chosen)
         }
         if ((rc = cgfd_init(c)) != 0) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     c->zchunks = best;
     c->args_dirty = true;
+    // then eager stepping against graph replay of 4-step cycles (one warm-up
+    // cycle captures the graph), when the run is long enough for both
+    if (!rc && trial_steps >= 4 && c->d.nt >= 8) {
+        float ms[2] = {0, 0};
+        for (int gr = 0; gr < 2 && !rc; gr++) {
+            if ((rc = cgfd_init(c)) != 0) break;
+            c->use_graph = gr == 1;
+            if ((rc = cgfd_run(c, 4)) != 0) break;
+            if ((rc = hipEventRecord(e0, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
+            if ((rc = cgfd_run(c, 4)) != 0) break;
+            if ((rc = hipEventRecord(e1, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[gr], e0, e1) != hipSuccess)
+                rc = fail(c, CGFD_EHIP, "event timing failed");
+        }
+        c->use_graph = !rc && ms[1] < ms[0];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (!rc) rc = cgfd_init(c);
+    if (!c->use_graph) drop_graphs(c);
     if (chosen) *chosen = best;
     return rc;
 }

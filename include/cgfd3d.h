@@ -128,7 +128,9 @@ int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
  * running each context alone. */
 int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few
- * z-chunk counts around the default, keep the fastest (*chosen), and
+ * z-chunk counts around the default and keep the fastest (*chosen); with
+ * trial_steps >= 4 and nt >= 8, also time 4 eager steps against 4 replayed
+ * ones and switch graph replay (cgfd_set_graph) on if it is faster.  Then
  * re-initialise the context.  Call right after cgfd_init. */
 int cgfd_autotune(cgfd_ctx *ctx, int trial_steps, int *chosen);
 /* Graph replay (no reference counterpart; for small grids whose steps are

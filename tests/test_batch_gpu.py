@@ -47,10 +47,11 @@ def test_batch_rejects_running_past_nt():
         cgfd3d.run_batch([g], 6)
 
 
-def test_autotune_keeps_results():
+@pytest.mark.parametrize("trial", [2, 4])   # 4: the graph-replay choice is timed too
+def test_autotune_keeps_results(trial):
     sc = scenario.config_random(48, 40, 36, 24, [5, 4, 4, 5, 5, 0], seed=76)
     g = _sim(sc)
-    z = g.autotune(trial_steps=2)
+    z = g.autotune(trial_steps=trial)
     assert 1 <= z <= sc["nk"] // 2
     assert g.step == 0
     g.run(24)

@@ -22,8 +22,11 @@ def _scenario(nt):
     return sc
 
 
-def _sim(sc):
-    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+def _sim(sc, double=False):
+    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"],
+                          precision=cgfd3d.FP64 if double else cgfd3d.FP32)
+    if double:
+        sc = dict(sc, **{k: sc[k].astype(np.float64) for k in ("alpha", "beta", "rho")})
     g.setup(sc)
     return g
 
@@ -41,14 +44,15 @@ def _outputs(g, sc):
     return out
 
 
-@pytest.mark.parametrize("chunks", [[23], [6, 9, 8], [1, 2, 3, 4, 5, 8]])
-def test_graph_replay_equals_eager(chunks):
+@pytest.mark.parametrize("chunks,double", [([23], False), ([6, 9, 8], False), ([1, 2, 3, 4, 5, 8], False),
+                                           ([6, 9, 8], True)])
+def test_graph_replay_equals_eager(chunks, double):
     nt = sum(chunks)
     sc = _scenario(nt)
-    e = _sim(sc)
+    e = _sim(sc, double)
     e.run(nt)
     want = _outputs(e, sc)
-    g = _sim(sc)
+    g = _sim(sc, double)
     g.set_graph(True)
     for n in chunks:
         g.run(n)

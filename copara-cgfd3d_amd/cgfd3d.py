@@ -55,6 +55,7 @@ SIGNATURES = {
     "cgfd_sync": (ctypes.c_int, [_VP]),
     "cgfd_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_batch": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
+    "cgfd_batch_fusable": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int]),
     "cgfd_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
     "cgfd_current_step": (ctypes.c_int, [_VP]),
     "cgfd_current_stage": (ctypes.c_int, [_VP]),
@@ -152,7 +153,8 @@ def nc_peakvel(path, vx, vy, vz):
 
 
 def run_batch(sims, nsteps):
-    """Advance several Simulations by nsteps together (one stream each)."""
+    """Advance several Simulations by nsteps together: one launch per kernel
+    for all of them when batch_fusable(sims), else one stream each."""
     if not sims:
         return
     lib = sims[0].lib
@@ -161,6 +163,20 @@ def run_batch(sims, nsteps):
     if rc:
         msgs = "; ".join(lib.cgfd_last_error(s.h).decode() for s in sims)
         raise CgfdError(f"cgfd_run_batch failed ({rc}): {msgs}")
+
+
+def batch_fusable(sims):
+    """True if run_batch advances these Simulations with one launch per
+    kernel (same device, precision and grid dimensions, default stage
+    variant, same metric mode and step phase, no kernel timing)."""
+    if not sims:
+        return False
+    lib = sims[0].lib
+    arr = (_VP * len(sims))(*[s.h for s in sims])
+    rc = lib.cgfd_batch_fusable(arr, len(sims))
+    if rc < 0:
+        raise CgfdError(f"cgfd_batch_fusable failed ({rc})")
+    return bool(rc)
 
 
 class Simulation:

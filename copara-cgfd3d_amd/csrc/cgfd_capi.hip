@@ -9,6 +9,7 @@
 #include "cgfd_kernels.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
 #include <cmath>
 #include <cstdio>
@@ -95,8 +96,15 @@ struct cgfd_ctx {
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     int rows = 4, zchunks = 1;
     int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
-    char *args_d = nullptr; // the 4 per-stage argument blocks in device memory
+    char *args_d = nullptr; // argument image (kCtxArgBytes: stage, traction-image and trace blocks)
     bool args_dirty = true;
+    // batches led by this context (the first of cgfd_run_batch's list):
+    // the argument images of the batch's grids end to end, its host copy,
+    // and an event that orders the other contexts' streams around it
+    char *bimg_d = nullptr;
+    size_t bimg_cap = 0;
+    std::vector<char> bimg_h;
+    hipEvent_t bev = nullptr;
 
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_stage, ev_timg;
@@ -334,6 +342,8 @@ void cgfd_destroy(cgfd_ctx *c)
     dfree(c->seis);
     free_snaps(c);
     dfree(c->args_d);
+    dfree(c->bimg_d);
+    if (c->bev) (void)hipEventDestroy(c->bev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -799,8 +809,6 @@ static void fold_events(cgfd_ctx *c)
 
 } // extern "C"
 
-static const int kStageArgsBytes = 1024; // room for one StageArgs<double>
-
 template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int k)
 {
     static const R RK4A[3] = {(R)0.5, (R)0.5, (R)1.0};                                      // calculate.cu:12
@@ -850,20 +858,66 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     return sa;
 }
 
-// (re)write the four per-stage argument blocks the stage kernels read
+template <typename R> static TimgArgs<R> make_timg_args(const cgfd_ctx *c, int k)
+{
+    const int sin_ = stage_in(k);
+    TimgArgs<R> ta{};
+    ta.g = c->g;
+    ta.win = (const R *)c->set_ptr(sin_);
+    ta.met = (const R *)c->met;
+    ta.med = (const R *)c->med;
+    ta.pml = (const R *)c->pml;
+    memcpy(ta.nabs, c->nabs, sizeof(ta.nabs));
+    for (int a = 0; a < 2; a++) {
+        ta.npml[a] = c->npml[a];
+        ta.astr[a] = c->astr[a];
+        ta.ain[a] = (const R *)c->aux_ptr(a, sin_);
+    }
+    ta.timg = (R *)c->timg;
+    ta.tstr = c->tstr;
+    ta.h = c->h;
+    return ta;
+}
+
+template <typename R> static RecArgs<R> make_rec_args(const cgfd_ctx *c)
+{
+    RecArgs<R> ra{};
+    ra.fw = (const R *)c->set_ptr(SET_FW);
+    ra.fs = c->g.fs;
+    ra.rlin = c->rlin;
+    ra.seis = (R *)c->seis;
+    ra.step = c->d_step;
+    ra.nrecv = c->nrecv;
+    ra.nt = c->d.nt;
+    return ra;
+}
+
+// the argument image of one grid (layout in cgfd_kernels.h); zlen overrides
+// the stage blocks' z-chunk length when positive
+template <typename R> static void make_image(const cgfd_ctx *c, char *dst, int zlen)
+{
+    static_assert(sizeof(StageArgs<R>) <= kStageArgsBytes && sizeof(TimgArgs<R>) <= kTimgArgsBytes &&
+                      sizeof(RecArgs<R>) <= kRecArgsBytes,
+                  "argument block too large");
+    static_assert(offsetof(RecArgs<float>, step) == offsetof(RecArgs<double>, step), "step counter offset");
+    memset(dst, 0, kCtxArgBytes);
+    for (int k = 0; k < 4; k++) {
+        StageArgs<R> a = make_stage_args<R>(c, k);
+        if (zlen > 0) a.zlen = zlen;
+        memcpy(dst + k * kStageArgsBytes, &a, sizeof(a));
+        TimgArgs<R> t = make_timg_args<R>(c, k);
+        memcpy(dst + kTimgArgsOffset + k * kTimgArgsBytes, &t, sizeof(t));
+    }
+    RecArgs<R> r = make_rec_args<R>(c);
+    memcpy(dst + kRecArgsOffset, &r, sizeof(r));
+}
+
+// (re)write the context's argument image
 static int upload_args(cgfd_ctx *c)
 {
-    static_assert(sizeof(StageArgs<double>) <= kStageArgsBytes, "argument block too large");
-    std::vector<char> host(4 * kStageArgsBytes, 0);
-    for (int k = 0; k < 4; k++) {
-        if (c->prec == CGFD_FP32) {
-            StageArgs<float> a = make_stage_args<float>(c, k);
-            memcpy(host.data() + k * kStageArgsBytes, &a, sizeof(a));
-        } else {
-            StageArgs<double> a = make_stage_args<double>(c, k);
-            memcpy(host.data() + k * kStageArgsBytes, &a, sizeof(a));
-        }
-    }
+    std::vector<char> host(kCtxArgBytes, 0);
+    if (c->prec == CGFD_FP32) make_image<float>(c, host.data(), 0);
+    else make_image<double>(c, host.data(), 0);
     if (!c->args_d) {
         int rc = dalloc(c, &c->args_d, host.size());
         if (rc) return rc;
@@ -881,29 +935,14 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     int sx, sz;
     fb_signs(c->currT, k, &sx, &sz);
     const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2); // Tindex = 2*currT + tinc, calculate.cu:1404-1421
-    const int sin_ = stage_in(k);
     if (c->args_dirty) {
         int rc = upload_args(c);
         if (rc) return rc;
     }
     const StageArgs<R> sa = make_stage_args<R>(c, k);
     const StageArgs<R> *pa = (const StageArgs<R> *)(c->args_d + k * kStageArgsBytes);
-
-    TimgArgs<R> ta{};
-    ta.g = c->g;
-    ta.win = (const R *)c->set_ptr(sin_);
-    ta.met = (const R *)c->met;
-    ta.med = (const R *)c->med;
-    ta.pml = (const R *)c->pml;
-    memcpy(ta.nabs, c->nabs, sizeof(ta.nabs));
-    for (int a = 0; a < 2; a++) {
-        ta.npml[a] = c->npml[a];
-        ta.astr[a] = c->astr[a];
-        ta.ain[a] = (const R *)c->aux_ptr(a, sin_);
-    }
-    ta.timg = (R *)c->timg;
-    ta.tstr = c->tstr;
-    ta.h = c->h;
+    const TimgArgs<R> ta = make_timg_args<R>(c, k);
+    const TimgArgs<R> *pt = (const TimgArgs<R> *)(c->args_d + kTimgArgsOffset + k * kTimgArgsBytes);
 
     // profiling: events bracket each launch on the context's stream
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -911,12 +950,12 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
-    launch_timg<R>(sx, sz, ta, c->rows, c->stream);
+    launch_timg<R>(sx, sz, ta, pt, 1, c->rows, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
     }
-    launch_stage<R>(c->variant, sx, sz, k, sa, pa, tinc, c->rows, c->zchunks, c->stream);
+    launch_stage<R>(c->variant, sx, sz, k, sa, pa, 1, tinc, c->rows, c->zchunks, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         c->ev_timg.push_back({ev[0], ev[1]});
@@ -1044,6 +1083,122 @@ int cgfd_set_graph(cgfd_ctx *c, int on)
 // is enqueued on that context's own stream before stage k+1 of any, so the
 // GPU overlaps small grids that would leave most CUs idle on their own, and
 // one context's launch gaps are filled by the others' kernels.
+} // extern "C"
+
+// A batch runs as one launch per kernel when its grids share a device, the
+// precision, the padded geometry, the LDS stage variant with the same metric
+// mode, and the step phase currT % 4 (which fixes the forward/backward
+// pattern of every stage), and no context is timing its kernels.
+static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
+{
+    if (n < 2 || n > 4096) return false; // 9 traction-image parts per grid in a 65535 grid.z
+    const cgfd_ctx *c0 = ctxs[0];
+    auto hm = [](const cgfd_ctx *c) { return c->hmet && c->use_hmet; };
+    for (int i = 0; i < n; i++) {
+        const cgfd_ctx *c = ctxs[i];
+        if (c->prof || c->variant != 0) return false;
+        if (i == 0) continue;
+        if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0) ||
+            memcmp(&c->g, &c0->g, sizeof(Geom)) != 0)
+            return false;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return false; // a grid listed twice would be stepped twice per launch
+    }
+    return true;
+}
+
+// z chunks of a batched launch: about CGFD_BATCH_WORKGROUPS workgroups of
+// 64x4 tiles over all grids, at least 2 planes per chunk
+#ifndef CGFD_BATCH_WORKGROUPS
+#define CGFD_BATCH_WORKGROUPS 2048
+#endif
+static int batch_zchunks(const cgfd_ctx *c0, int n)
+{
+    const long tiles = (long)((c0->g.ni + 63) / 64) * ((c0->g.nj + 3) / 4) * n;
+    long zc = (CGFD_BATCH_WORKGROUPS + tiles - 1) / tiles;
+    return (int)std::max(1L, std::min(zc, (long)std::max(1, c0->g.nk / 2)));
+}
+
+template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, int nsteps)
+{
+    cgfd_ctx *c0 = ctxs[0];
+    HIPCHK(c0, hipSetDevice(c0->dev));
+    for (int i = 0; i < n; i++)
+        if (ctxs[i]->args_dirty) {
+            int rc = upload_args(ctxs[i]);
+            if (rc) return rc;
+        }
+    const int zc = batch_zchunks(c0, n), zlen = (c0->g.nk + zc - 1) / zc;
+    std::vector<char> img((size_t)n * kCtxArgBytes);
+    int max_nrecv = 0;
+    for (int i = 0; i < n; i++) {
+        make_image<R>(ctxs[i], img.data() + (size_t)i * kCtxArgBytes, zlen);
+        max_nrecv = std::max(max_nrecv, ctxs[i]->nrecv);
+    }
+    if (img != c0->bimg_h) {
+        // launches of an earlier batch on c0's stream may still read the image
+        HIPCHK(c0, hipStreamSynchronize(c0->stream));
+        if (c0->bimg_cap < img.size()) {
+            dfree(c0->bimg_d);
+            c0->bimg_cap = 0;
+            int rc = dalloc(c0, &c0->bimg_d, img.size());
+            if (rc) return rc;
+            c0->bimg_cap = img.size();
+        }
+        HIPCHK(c0, hipMemcpy(c0->bimg_d, img.data(), img.size(), hipMemcpyHostToDevice));
+        c0->bimg_h.swap(img);
+    }
+    // c0's stream runs the batch: it waits for work queued on the others
+    for (int i = 0; i < n; i++) {
+        cgfd_ctx *c = ctxs[i];
+        if (!c->bev) HIPCHK(c, hipEventCreateWithFlags(&c->bev, hipEventDisableTiming));
+        if (i == 0) continue;
+        HIPCHK(c, hipEventRecord(c->bev, c->stream));
+        HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->bev, 0));
+    }
+    const char *img_d = c0->bimg_d;
+    const StageArgs<R> sa0 = make_stage_args<R>(c0, 0);
+    const TimgArgs<R> ta0 = make_timg_args<R>(c0, 0);
+    for (int t = 0; t < nsteps; t++) {
+        for (int k = 0; k < 4; k++) {
+            int sx, sz;
+            fb_signs(c0->currT, k, &sx, &sz);
+            const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
+            launch_timg<R>(sx, sz, ta0, (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n,
+                           c0->rows, c0->stream);
+            launch_stage<R>(0, sx, sz, k, sa0, (const StageArgs<R> *)(img_d + k * kStageArgsBytes), n, tinc,
+                            c0->rows, zc, c0->stream);
+        }
+        launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
+        for (int i = 0; i < n; i++) {
+            cgfd_ctx *c = ctxs[i];
+            for (auto &sn : c->snaps) {
+                const int tinv = sn.spec[9];
+                if (c->currT % tinv != 0 || c->currT / tinv >= sn.ntime) continue;
+                launch_snap<R>((const R *)c->set_ptr(SET_FW), c->g.fs, sn.lin, sn.npts, sn.c0, sn.ncomp,
+                               (R *)sn.buf, c->d_step, tinv, sn.ntime, c0->stream);
+            }
+        }
+        launch_step_advance_batch(img_d + kRecArgsOffset, n, c0->stream);
+        HIPCHK(c0, hipGetLastError());
+        for (int i = 0; i < n; i++) ctxs[i]->currT++;
+    }
+    // the other streams wait for the batch before their own later work
+    HIPCHK(c0, hipEventRecord(c0->bev, c0->stream));
+    for (int i = 1; i < n; i++) HIPCHK(ctxs[i], hipStreamWaitEvent(ctxs[i]->stream, c0->bev, 0));
+    return 0;
+}
+
+extern "C" {
+
+int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n)
+{
+    if (!ctxs || n < 0) return CGFD_EINVAL;
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i]) return CGFD_EINVAL;
+    return batch_fusable(ctxs, n) ? 1 : 0;
+}
+
 int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
 {
     if (!ctxs || n < 0 || nsteps < 0) return CGFD_EINVAL;
@@ -1055,6 +1210,9 @@ int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
         if ((long)c->currT + nsteps > c->d.nt)
             return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
     }
+    if (nsteps > 0 && batch_fusable(ctxs, n))
+        return ctxs[0]->prec == CGFD_FP32 ? run_batch_fused<float>(ctxs, n, nsteps)
+                                          : run_batch_fused<double>(ctxs, n, nsteps);
     bool graphs = n > 0;
     for (int i = 0; i < n; i++) graphs = graphs && ctxs[i]->use_graph && !ctxs[i]->prof;
     if (graphs) {

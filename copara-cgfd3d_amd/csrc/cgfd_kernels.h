@@ -59,14 +59,40 @@ template <typename R> struct TimgArgs {
     double h;
 };
 
+// Trace gather of one grid after a step (the batched form reads one per grid).
+template <typename R> struct RecArgs {
+    const R *fw;
+    long fs;
+    const long *rlin;
+    R *seis;
+    int *step;
+    int nrecv, nt;
+};
+
+// Argument image of one grid in device memory: the 4 per-stage StageArgs
+// blocks, the 4 per-stage TimgArgs blocks and the RecArgs block.  A batch
+// lays the images of its grids end to end, so grid c's blocks sit at
+// c * kCtxArgBytes from grid 0's.
+constexpr int kStageArgsBytes = 1024, kTimgArgsBytes = 512, kRecArgsBytes = 64;
+constexpr int kTimgArgsOffset = 4 * kStageArgsBytes;
+constexpr int kRecArgsOffset = kTimgArgsOffset + 4 * kTimgArgsBytes;
+constexpr int kCtxArgBytes = 8192;
+static_assert(kRecArgsOffset + kRecArgsBytes <= kCtxArgBytes, "argument image too small");
+
 // variant 0: LDS-staged xy planes (default); 1: direct loads of the in-plane
-// taps.  `a` is the host copy of the argument block (launch geometry), `pa`
-// the same block in device memory, read by the kernel.  tinc = 0, 1, 1, 2 in
-// stages 0-3: the kernel forms Tindex = 2*(*a.step) + tinc.
+// taps; 2: split velocity/stress kernels.  `a` is the host copy of grid 0's
+// argument block (launch geometry), `pa` the same block in device memory,
+// read by the kernel.  With nctx > 1 (variant 0 only) one launch advances
+// the same stage of nctx grids of identical geometry and metric mode, whose
+// blocks follow at kCtxArgBytes strides: workgroup w works on grid
+// w / (workgroups per grid).  tinc = 0, 1, 1, 2 in stages 0-3: the kernel
+// forms Tindex = 2*(*a.step) + tinc.
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
-                  int rows, int zchunks, hipStream_t s);
-template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s);
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx,
+                  int tinc, int rows, int zchunks, hipStream_t s);
+// traction image of nctx grids (blocks at kCtxArgBytes strides from pa)
+template <typename R>
+void launch_timg(int sx, int sz, const TimgArgs<R> &a, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s);
 template <typename R>
 void launch_velpd(const Geom &g, const R *met, const R *med, const R *pml, R *cov, long cstr, hipStream_t s);
 template <typename R>
@@ -87,5 +113,10 @@ void launch_snap(const R *fw, long fs, const long *lin, int npts, int c0, int nc
                  int tinv, int ntime, hipStream_t s);
 // step counter += 1 (one thread)
 void launch_step_advance(int *step, hipStream_t s);
+// The batched forms read the RecArgs blocks of nctx argument images from pa:
+// trace gathers of every grid (grids without receivers return at once), and
+// every grid's step counter += 1.
+template <typename R> void launch_record_batch(const RecArgs<R> *pa, int nctx, int max_nrecv, hipStream_t s);
+void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s);
 
 } // namespace cgfd

@@ -583,13 +583,17 @@ template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
 template <typename R, int SX, int SZ, int ST, int HM>
-__global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kernel(const StageArgs<R> *pa, int tinc)
+__global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR
+stage_lds_kernel(const StageArgs<R> *pa, int nblk, int tinc)
 {
     // The argument block lives in device memory and is read through the
     // constant address space, so fields are fetched with scalar loads where
     // they are used instead of being preloaded into (and spilled from) SGPRs.
+    // A batched launch covers several grids of one geometry, nblk workgroups
+    // each, whose blocks follow at kCtxArgBytes strides.
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
-    const AP a0 = (AP)pa;
+    const int ctx = blockIdx.x / nblk;
+    const AP a0 = (AP)((const char *)pa + (long)ctx * kCtxArgBytes);
     const int tindex = 2 * *a0->step + tinc;
     constexpr int TY = lds_ty<R>(), HX = 64 + 4, HY = TY + 4;
     constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
@@ -620,7 +624,7 @@ __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR stage_lds_kern
     // 8*CGFD_TILE_GROUP consecutive tiles.  Both cut the L2 fetch traffic by
     // ~8 % on the 600x600x300 model yet run slower than the identity map.
     const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
-    const int nb = gridDim.x, id = blockIdx.x;
+    const int nb = nblk, id = blockIdx.x - ctx * nblk;
 #ifndef CGFD_TILE_MAP
 #define CGFD_TILE_MAP 1
 #endif
@@ -1185,8 +1189,12 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
 // traction image, top three rows (CalTIMG, calculate.cu:5428-5666)
 // ---------------------------------------------------------------------------
 template <typename R, int SX, int SZ, int SPLIT>
-__global__ void __launch_bounds__(256) timg_kernel(TimgArgs<R> a)
+__global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> *__restrict__ pa)
 {
+    // blockIdx.z = grid * SPLIT + part: grid picks the argument block of a
+    // batch (kCtxArgBytes apart), part the split of the nine row equations
+    const int ctx = blockIdx.z / SPLIT, bz = blockIdx.z - ctx * SPLIT;
+    const TimgArgs<R> &a = *(const TimgArgs<R> *)((const char *)pa + (long)ctx * kCtxArgBytes);
     const Geom g = a.g;
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1212,7 +1220,7 @@ __global__ void __launch_bounds__(256) timg_kernel(TimgArgs<R> a)
     // grids keep all nine per thread (SPLIT = 1), which shares the loads of a
     // column.  The loops stay unrolled, so r and m are compile-time constants
     // in every copy.
-    const int rz = SPLIT == 9 ? blockIdx.z / 3 : blockIdx.z, mz = blockIdx.z % 3;
+    const int rz = SPLIT == 9 ? bz / 3 : bz, mz = bz % 3;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         if (SPLIT > 1 && r != rz) continue;
@@ -1474,16 +1482,18 @@ __global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, int SX, int SZ, int ST>
-static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc, int rows,
-                             int zchunks, hipStream_t s)
+static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx, int tinc,
+                             int rows, int zchunks, hipStream_t s)
 {
     if (variant == 0) {
         constexpr int ty = lds_ty<R>();
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
+        const int nblk = ntx * nty * zchunks;
+        const dim3 grid((unsigned)((long)nblk * nctx));
         if (a.hmet)
-            stage_lds_kernel<R, SX, SZ, ST, 1><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tinc);
+            stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, nblk, tinc);
         else
-            stage_lds_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, ty), 0, s>>>(pa, tinc);
+            stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, nblk, tinc);
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
         stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tinc);
@@ -1495,45 +1505,46 @@ static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs
 }
 
 template <typename R, int SX, int SZ>
-static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc, int rows,
-                             int zchunks, hipStream_t s)
+static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx, int tinc,
+                             int rows, int zchunks, hipStream_t s)
 {
     switch (st) {
-    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, tinc, rows, zchunks, s); break;
-    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, tinc, rows, zchunks, s); break;
-    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, tinc, rows, zchunks, s); break;
-    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, tinc, rows, zchunks, s); break;
+    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
+    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
+    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
+    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
     }
 }
 
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
-                  int rows, int zchunks, hipStream_t s)
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx,
+                  int tinc, int rows, int zchunks, hipStream_t s)
 {
-    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, tinc, rows, zchunks, s);
-    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, tinc, rows, zchunks, s);
-    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, tinc, rows, zchunks, s);
-    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, tinc, rows, zchunks, s);
+    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
+    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
+    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
+    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
 }
 
 template <typename R, int SPLIT>
-static void launch_timg_split(int sx, int sz, const TimgArgs<R> &a, dim3 grid, dim3 block, hipStream_t s)
+static void launch_timg_split(int sx, int sz, const TimgArgs<R> *pa, dim3 grid, dim3 block, hipStream_t s)
 {
-    if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT><<<grid, block, 0, s>>>(a);
-    else if (sx > 0) timg_kernel<R, 1, -1, SPLIT><<<grid, block, 0, s>>>(a);
-    else if (sz > 0) timg_kernel<R, -1, 1, SPLIT><<<grid, block, 0, s>>>(a);
-    else timg_kernel<R, -1, -1, SPLIT><<<grid, block, 0, s>>>(a);
+    if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT><<<grid, block, 0, s>>>(pa);
+    else if (sx > 0) timg_kernel<R, 1, -1, SPLIT><<<grid, block, 0, s>>>(pa);
+    else if (sz > 0) timg_kernel<R, -1, 1, SPLIT><<<grid, block, 0, s>>>(pa);
+    else timg_kernel<R, -1, -1, SPLIT><<<grid, block, 0, s>>>(pa);
 }
 
-template <typename R> void launch_timg(int sx, int sz, const TimgArgs<R> &a, int rows, hipStream_t s)
+template <typename R>
+void launch_timg(int sx, int sz, const TimgArgs<R> &a, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s)
 {
     dim3 block(64, rows);
-    const long cols = (long)a.g.ni * a.g.nj;
+    const long cols = (long)a.g.ni * a.g.nj * nctx;
     const int split = cols < 65536 ? 9 : (cols < 131072 ? 3 : 1); // see timg_kernel
-    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, split);
-    if (split == 9) launch_timg_split<R, 9>(sx, sz, a, grid, block, s);
-    else if (split == 3) launch_timg_split<R, 3>(sx, sz, a, grid, block, s);
-    else launch_timg_split<R, 1>(sx, sz, a, grid, block, s);
+    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, split * nctx);
+    if (split == 9) launch_timg_split<R, 9>(sx, sz, pa, grid, block, s);
+    else if (split == 3) launch_timg_split<R, 3>(sx, sz, pa, grid, block, s);
+    else launch_timg_split<R, 1>(sx, sz, pa, grid, block, s);
 }
 
 template <typename R>
@@ -1592,10 +1603,47 @@ __global__ void step_advance_kernel(int *step)
 
 void launch_step_advance(int *step, hipStream_t s) { step_advance_kernel<<<1, 64, 0, s>>>(step); }
 
+// batched trace gathers: blockIdx.y = grid, whose RecArgs block is
+// kCtxArgBytes from the previous grid's
+template <typename R> __global__ void record_batch_kernel(const RecArgs<R> *__restrict__ pa)
+{
+    const RecArgs<R> &a = *(const RecArgs<R> *)((const char *)pa + (long)blockIdx.y * kCtxArgBytes);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nrecv * NF) return;
+    const int t = *a.step;
+    if (t >= a.nt) return;
+    const int r = i / NF, c = i % NF;
+    a.seis[((long)c * a.nrecv + r) * a.nt + t] = a.fw[c * a.fs + a.rlin[r]];
+}
+
+template <typename R> void launch_record_batch(const RecArgs<R> *pa, int nctx, int max_nrecv, hipStream_t s)
+{
+    if (max_nrecv <= 0 || nctx <= 0) return;
+    const int n = max_nrecv * NF;
+    record_batch_kernel<R><<<dim3((n + 255) / 256, nctx), 256, 0, s>>>(pa);
+}
+
+// every grid's step counter += 1: lane i reads grid i's counter address from
+// its RecArgs block (the same offset in both precisions)
+__global__ void step_advance_batch_kernel(const char *__restrict__ pa, int nctx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nctx) return;
+    int *step = ((const RecArgs<float> *)(pa + (long)i * kCtxArgBytes))->step;
+    step[0] += 1;
+}
+
+void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s)
+{
+    if (nctx <= 0) return;
+    step_advance_batch_kernel<<<(nctx + 63) / 64, 64, 0, s>>>(pa, nctx);
+}
+
 #define CGFD_INSTANTIATE(R)                                                                                   \
-    template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int,  \
-                                   hipStream_t);              \
-    template void launch_timg<R>(int, int, const TimgArgs<R> &, int, hipStream_t);                          \
+    template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int, \
+                                   int, hipStream_t);                                                         \
+    template void launch_timg<R>(int, int, const TimgArgs<R> &, const TimgArgs<R> *, int, int, hipStream_t); \
+    template void launch_record_batch<R>(const RecArgs<R> *, int, int, hipStream_t);                          \
     template void launch_velpd<R>(const Geom &, const R *, const R *, const R *, R *, long, hipStream_t);     \
     template void launch_to_padded<R>(const R *, R *, const Geom &, hipStream_t);                            \
     template void launch_from_padded<R>(const R *, R *, const Geom &, hipStream_t);                          \

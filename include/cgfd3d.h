@@ -123,10 +123,21 @@ int cgfd_run(cgfd_ctx *ctx, int nsteps);
 int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
 /* Batch of independent simulations (no reference counterpart; the
  * reference runs one scenario per job): advance each of the n contexts by
- * nsteps steps, enqueueing stage k of every context on its own stream before
- * stage k+1 of any, so small grids share the GPU.  Results equal those of
- * running each context alone. */
+ * nsteps steps.  A fusable list (cgfd_batch_fusable) runs as one launch per
+ * kernel and stage for all its grids; otherwise stage k of every context is
+ * enqueued on its own stream before stage k+1 of any, so small grids share
+ * the GPU.  Results equal those of running each context alone. */
 int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
+/* 1 if cgfd_run_batch runs this list as one launch per kernel (no reference
+ * counterpart): 2 to 4096 distinct contexts on one device with the same
+ * precision and grid dimensions (hence one padded layout; media, PML,
+ * sources and receivers may differ), the default stage variant, the same
+ * metric mode and the same step phase (step % 4), none timing its kernels.
+ * Such a batch runs on the first context's stream, which waits for the
+ * others' queued work, with its own z-chunk count (the contexts' tuning is
+ * left as it is); the other streams wait for the batch.  Otherwise 0 (or
+ * CGFD_EINVAL for a null list). */
+int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few
  * z-chunk counts around the default and keep the fastest (*chosen); with
  * trial_steps >= 4 and nt >= 8, also time 4 eager steps against 4 replayed

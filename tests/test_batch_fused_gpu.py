@@ -1,0 +1,132 @@
+"""Batched launches (cgfd_run_batch on a fusable list): grids of one size
+advanced by one launch per kernel and stage give, value for value, the
+traces, PGV, snapshots and final state of running each grid alone, also
+with different media, PML widths, sources and receiver counts per grid,
+with single runs interleaved between batches, and in FP64.  Lists that
+cannot share a launch (other sizes, another step phase, a repeated
+context, another stage variant, kernel timing) are reported as such and
+still run correctly on one stream per grid."""
+import numpy as np
+import pytest
+
+from conftest import assert_same
+
+pytestmark = pytest.mark.gpu
+
+cgfd3d = pytest.importorskip("cgfd3d")
+import scenario  # noqa: E402
+
+NI, NJ, NK = 36, 30, 20
+PMLS = ([4, 4, 3, 3, 4, 0], [2, 5, 4, 2, 3, 0], [0, 0, 0, 0, 0, 0])
+NRECV = (6, 2, 9)
+
+
+def _scenarios(nt, n=3, ni=NI):
+    scs = []
+    for i in range(n):
+        sc = scenario.config_random(ni, NJ, NK, nt, PMLS[i % 3], seed=90 + i, nfoc=2 if i == 1 else 0,
+                                    nrecv=NRECV[i % 3])
+        sc["snapshots"] = np.array([[1, 1, NK, ni, NJ, 1, 1, 1, 1, 3 + i % 3, 1],
+                                    [2 + i, 2, 2, 4, 3, 5, 5, 6, 3, 1, 3]], np.int32)
+        scs.append(sc)
+    return scs
+
+
+def _sim(sc, double=False):
+    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"],
+                          precision=cgfd3d.FP64 if double else cgfd3d.FP32)
+    if double:
+        sc = dict(sc, **{k: sc[k].astype(np.float64) for k in ("alpha", "beta", "rho")})
+    g.setup(sc)
+    return g
+
+
+def _outputs(g, sc):
+    out = {"seismo": g.get_seismo(), "pgv": g.get_pgv()[:, 3:-3, 3:-3]}
+    for i in range(len(sc["snapshots"])):
+        for c in range(9):
+            try:
+                out[f"snap{i}.{c}"] = g.get_snap(i, c)
+            except cgfd3d.CgfdError:
+                pass
+    for c in range(9):
+        out[f"field{c}"] = g.get_field(c)
+    return out
+
+
+def _check_against_single(batch, scs, nt, double=False):
+    for n, (b, sc) in enumerate(zip(batch, scs)):
+        s = _sim(sc, double)
+        s.run(nt)
+        want, got = _outputs(s, sc), _outputs(b, sc)
+        assert b.step == nt
+        for k in want:
+            assert_same(got[k], want[k], f"grid {n} {k}")
+
+
+@pytest.mark.parametrize("chunks,double", [([6, 9, 8], False), ([23], False), ([5, 7], True)])
+def test_fused_batch_equals_single_runs(chunks, double):
+    nt = sum(chunks)
+    scs = _scenarios(nt)
+    batch = [_sim(sc, double) for sc in scs]
+    assert cgfd3d.batch_fusable(batch)
+    for n in chunks:
+        cgfd3d.run_batch(batch, n)
+        batch[1].get_field(0)  # a read between batches
+    _check_against_single(batch, scs, nt, double)
+
+
+def test_fused_batch_interleaved_with_single_runs():
+    """Single runs of every grid (on its own stream) between batched
+    launches (on the first grid's stream): the streams wait for each other."""
+    scs = _scenarios(20)
+    batch = [_sim(sc) for sc in scs]
+    cgfd3d.run_batch(batch, 4)
+    for b in batch:
+        b.run(8)
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 8)
+    _check_against_single(batch, scs, 20)
+
+
+def test_fused_batch_with_graphs_on():
+    """A fusable list runs batched even when its grids replay graphs alone."""
+    scs = _scenarios(14)
+    batch = [_sim(sc) for sc in scs]
+    for b in batch:
+        b.set_graph(True)
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 14)
+    _check_against_single(batch, scs, 14)
+
+
+def test_fused_batch_many_grids():
+    scs = _scenarios(9, n=12)
+    batch = [_sim(sc) for sc in scs]
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 9)
+    _check_against_single(batch, scs, 9)
+
+
+def test_unfusable_lists_are_reported_and_still_run():
+    scs = _scenarios(12)
+    batch = [_sim(sc) for sc in scs]
+    assert not cgfd3d.batch_fusable(batch[:1])
+    assert not cgfd3d.batch_fusable([batch[0], batch[1], batch[0]])
+    other = _sim(_scenarios(12, n=1, ni=NI + 4)[0])
+    assert not cgfd3d.batch_fusable([batch[0], other])
+    batch[2].set_variant(1)
+    assert not cgfd3d.batch_fusable(batch)
+    batch[2].set_variant(0)
+    batch[0].profile(True)
+    assert not cgfd3d.batch_fusable(batch)
+    batch[0].profile(False)
+    assert cgfd3d.batch_fusable(batch)
+    # another step phase: one stream per grid
+    batch[0].run(1)
+    assert not cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch[1:], 1)
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 11)
+    _check_against_single(batch, scs, 12)
+    other.close()

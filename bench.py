@@ -68,8 +68,9 @@ def describe(sc):
 def batch_bench(args):
     """Batched FD: `--batch` independent hill-medium scenarios of uneven size
     (96-192 x 96-192 x 64-128 cells), advanced together with cgfd_run_batch
-    (one stream each, per-grid autotuned z-chunks), against the same
-    scenarios run one after another.  Prints one JSON line."""
+    (one launch per kernel and stage for all of them), against the same
+    scenarios run one after another with per-grid autotuned z-chunks.
+    Prints one JSON line."""
     import cgfd3d
     import scenario
 
@@ -105,6 +106,7 @@ def batch_bench(args):
 
     t_seq = timed(one_by_one)
     t_bat = timed(lambda n: cgfd3d.run_batch(sims, n))
+    fused = cgfd3d.batch_fusable(sims)
     for g in sims:
         g.close()
     value = cells * 4 * args.steps / t_bat / 1e9
@@ -112,7 +114,7 @@ def batch_bench(args):
         "metric": "Gcell-RKstages/s", "value": round(value, 4), "unit": "Gcell-RKstages/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_bat / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"batch-{len(scs)}-hill", "graph_replay": bool(args.graph),
+        "config": {"workload": f"batch-{len(scs)}-hill", "graph_replay": bool(args.graph), "fused": fused,
                    "scenarios": [sc["name"] for sc in scs],
                    "zchunks": zch, "cells_total": cells,
                    "scenario_steps_per_s_batched": round(len(scs) * args.steps / t_bat, 2),

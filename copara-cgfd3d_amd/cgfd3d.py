@@ -167,8 +167,8 @@ def run_batch(sims, nsteps):
 
 def batch_fusable(sims):
     """True if run_batch advances these Simulations with one launch per
-    kernel (same device, precision and grid dimensions, default stage
-    variant, same metric mode and step phase, no kernel timing)."""
+    kernel (same device and precision, default stage variant, same metric
+    mode and step phase, no kernel timing; sizes may differ)."""
     if not sims:
         return False
     lib = sims[0].lib

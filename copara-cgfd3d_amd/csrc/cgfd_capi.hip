@@ -941,7 +941,6 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     }
     const StageArgs<R> sa = make_stage_args<R>(c, k);
     const StageArgs<R> *pa = (const StageArgs<R> *)(c->args_d + k * kStageArgsBytes);
-    const TimgArgs<R> ta = make_timg_args<R>(c, k);
     const TimgArgs<R> *pt = (const TimgArgs<R> *)(c->args_d + kTimgArgsOffset + k * kTimgArgsBytes);
 
     // profiling: events bracket each launch on the context's stream
@@ -950,12 +949,12 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
-    launch_timg<R>(sx, sz, ta, pt, 1, c->rows, c->stream);
+    launch_timg<R>(sx, sz, c->g.ni, c->g.nj, (long)c->g.ni * c->g.nj, pt, 1, c->rows, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
     }
-    launch_stage<R>(c->variant, sx, sz, k, sa, pa, 1, tinc, c->rows, c->zchunks, c->stream);
+    launch_stage<R>(c->variant, sx, sz, k, sa, pa, tinc, c->rows, c->zchunks, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         c->ev_timg.push_back({ev[0], ev[1]});
@@ -1086,37 +1085,50 @@ int cgfd_set_graph(cgfd_ctx *c, int on)
 } // extern "C"
 
 // A batch runs as one launch per kernel when its grids share a device, the
-// precision, the padded geometry, the LDS stage variant with the same metric
-// mode, and the step phase currT % 4 (which fixes the forward/backward
-// pattern of every stage), and no context is timing its kernels.
+// precision, the LDS stage variant with the same metric mode, and the step
+// phase currT % 4 (which fixes the forward/backward pattern of every
+// stage), and no context is timing its kernels.  Grid sizes may differ.
+static long batch_tiles(const cgfd_ctx *c) { return (long)((c->g.ni + 63) / 64) * ((c->g.nj + 3) / 4); }
+
 static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 {
     if (n < 2 || n > 4096) return false; // 9 traction-image parts per grid in a 65535 grid.z
     const cgfd_ctx *c0 = ctxs[0];
     auto hm = [](const cgfd_ctx *c) { return c->hmet && c->use_hmet; };
+    long most = 0; // workgroups at the most z chunks batch_layout can pick
     for (int i = 0; i < n; i++) {
         const cgfd_ctx *c = ctxs[i];
         if (c->prof || c->variant != 0) return false;
+        most += batch_tiles(c) * std::max(1, c->g.nk / 2);
         if (i == 0) continue;
-        if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0) ||
-            memcmp(&c->g, &c0->g, sizeof(Geom)) != 0)
+        if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0))
             return false;
         for (int j = 0; j < i; j++)
             if (ctxs[j] == c) return false; // a grid listed twice would be stepped twice per launch
     }
-    return true;
+    return most < 0x7fffffffL;
 }
 
-// z chunks of a batched launch: about CGFD_BATCH_WORKGROUPS workgroups of
-// 64x4 tiles over all grids, at least 2 planes per chunk
+// Workgroups of a batched stage launch: every grid sweeps its columns in the
+// same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS workgroups
+// of 64x4 tiles over all grids and clamped to 2 planes per chunk per grid.
+// zlen[c] is grid c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
 #endif
-static int batch_zchunks(const cgfd_ctx *c0, int n)
+static void batch_layout(cgfd_ctx *const *ctxs, int n, std::vector<int> &zlen, std::vector<int> &starts)
 {
-    const long tiles = (long)((c0->g.ni + 63) / 64) * ((c0->g.nj + 3) / 4) * n;
-    long zc = (CGFD_BATCH_WORKGROUPS + tiles - 1) / tiles;
-    return (int)std::max(1L, std::min(zc, (long)std::max(1, c0->g.nk / 2)));
+    long tiles = 0;
+    for (int i = 0; i < n; i++) tiles += batch_tiles(ctxs[i]);
+    const long zc = std::max(1L, (CGFD_BATCH_WORKGROUPS + tiles - 1) / tiles);
+    zlen.assign(n, 0);
+    starts.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        const int nk = ctxs[i]->g.nk;
+        const int zci = (int)std::min(zc, (long)std::max(1, nk / 2));
+        zlen[i] = (nk + zci - 1) / zci;
+        starts[i + 1] = starts[i] + (int)(batch_tiles(ctxs[i]) * zci);
+    }
 }
 
 template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, int nsteps)
@@ -1128,13 +1140,22 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             int rc = upload_args(ctxs[i]);
             if (rc) return rc;
         }
-    const int zc = batch_zchunks(c0, n), zlen = (c0->g.nk + zc - 1) / zc;
-    std::vector<char> img((size_t)n * kCtxArgBytes);
-    int max_nrecv = 0;
+    // device image: the grids' argument images, then the workgroup starts
+    std::vector<int> zlen, starts;
+    batch_layout(ctxs, n, zlen, starts);
+    const size_t sofs = (size_t)n * kCtxArgBytes;
+    std::vector<char> img(sofs + starts.size() * sizeof(int));
+    int max_nrecv = 0, ni = 0, nj = 0;
+    long cols = 0;
     for (int i = 0; i < n; i++) {
-        make_image<R>(ctxs[i], img.data() + (size_t)i * kCtxArgBytes, zlen);
-        max_nrecv = std::max(max_nrecv, ctxs[i]->nrecv);
+        const cgfd_ctx *c = ctxs[i];
+        make_image<R>(c, img.data() + (size_t)i * kCtxArgBytes, zlen[i]);
+        max_nrecv = std::max(max_nrecv, c->nrecv);
+        ni = std::max(ni, c->g.ni);
+        nj = std::max(nj, c->g.nj);
+        cols += (long)c->g.ni * c->g.nj;
     }
+    memcpy(img.data() + sofs, starts.data(), starts.size() * sizeof(int));
     if (img != c0->bimg_h) {
         // launches of an earlier batch on c0's stream may still read the image
         HIPCHK(c0, hipStreamSynchronize(c0->stream));
@@ -1157,17 +1178,17 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
         HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->bev, 0));
     }
     const char *img_d = c0->bimg_d;
-    const StageArgs<R> sa0 = make_stage_args<R>(c0, 0);
-    const TimgArgs<R> ta0 = make_timg_args<R>(c0, 0);
+    const int *starts_d = (const int *)(img_d + sofs);
+    const int hmet = c0->hmet && c0->use_hmet ? 1 : 0;
     for (int t = 0; t < nsteps; t++) {
         for (int k = 0; k < 4; k++) {
             int sx, sz;
             fb_signs(c0->currT, k, &sx, &sz);
             const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
-            launch_timg<R>(sx, sz, ta0, (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n,
-                           c0->rows, c0->stream);
-            launch_stage<R>(0, sx, sz, k, sa0, (const StageArgs<R> *)(img_d + k * kStageArgsBytes), n, tinc,
-                            c0->rows, zc, c0->stream);
+            launch_timg<R>(sx, sz, ni, nj, cols, (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes),
+                           n, 4, c0->stream);
+            launch_stage_batch<R>(sx, sz, k, hmet, (const StageArgs<R> *)(img_d + k * kStageArgsBytes), starts_d, n,
+                                  starts[n], tinc, c0->stream);
         }
         launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
         for (int i = 0; i < n; i++) {

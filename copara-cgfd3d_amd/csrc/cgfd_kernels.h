@@ -80,19 +80,24 @@ constexpr int kCtxArgBytes = 8192;
 static_assert(kRecArgsOffset + kRecArgsBytes <= kCtxArgBytes, "argument image too small");
 
 // variant 0: LDS-staged xy planes (default); 1: direct loads of the in-plane
-// taps; 2: split velocity/stress kernels.  `a` is the host copy of grid 0's
+// taps; 2: split velocity/stress kernels.  `a` is the host copy of the
 // argument block (launch geometry), `pa` the same block in device memory,
-// read by the kernel.  With nctx > 1 (variant 0 only) one launch advances
-// the same stage of nctx grids of identical geometry and metric mode, whose
-// blocks follow at kCtxArgBytes strides: workgroup w works on grid
-// w / (workgroups per grid).  tinc = 0, 1, 1, 2 in stages 0-3: the kernel
-// forms Tindex = 2*(*a.step) + tinc.
+// read by the kernel.  tinc = 0, 1, 1, 2 in stages 0-3: the kernel forms
+// Tindex = 2*(*a.step) + tinc.
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx,
-                  int tinc, int rows, int zchunks, hipStream_t s);
-// traction image of nctx grids (blocks at kCtxArgBytes strides from pa)
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
+                  int rows, int zchunks, hipStream_t s);
+// The same stage of nctx grids in one launch of the LDS variant: grid c's
+// argument block lies c * kCtxArgBytes after pa and it owns workgroups
+// starts[c] .. starts[c+1]-1 (starts in device memory, starts[nctx] =
+// nblocks); all grids share the step phase and the metric mode hmet.
 template <typename R>
-void launch_timg(int sx, int sz, const TimgArgs<R> &a, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s);
+void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
+                        int nblocks, int tinc, hipStream_t s);
+// traction image of nctx grids (blocks at kCtxArgBytes strides from pa);
+// ni x nj covers every grid's surface, cols is their total column count
+template <typename R>
+void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s);
 template <typename R>
 void launch_velpd(const Geom &g, const R *met, const R *med, const R *pml, R *cov, long cstr, hipStream_t s);
 template <typename R>

@@ -584,15 +584,29 @@ template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 
 
 template <typename R, int SX, int SZ, int ST, int HM>
 __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR
-stage_lds_kernel(const StageArgs<R> *pa, int nblk, int tinc)
+stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 {
     // The argument block lives in device memory and is read through the
     // constant address space, so fields are fetched with scalar loads where
     // they are used instead of being preloaded into (and spilled from) SGPRs.
-    // A batched launch covers several grids of one geometry, nblk workgroups
-    // each, whose blocks follow at kCtxArgBytes strides.
+    // A batched launch (nctx > 1) covers several grids: grid c owns
+    // workgroups starts[c] .. starts[c+1]-1 and its block lies c *
+    // kCtxArgBytes after pa.  The search runs on scalar registers.
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
-    const int ctx = blockIdx.x / nblk;
+    typedef const __attribute__((address_space(4))) int *SP;
+    int ctx = 0, nb = gridDim.x, id = blockIdx.x;
+    if (nctx > 1) {
+        const SP st = (SP)starts;
+        int lo = 0, hi = nctx - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (st[mid] <= id) lo = mid;
+            else hi = mid - 1;
+        }
+        ctx = lo;
+        nb = st[lo + 1] - st[lo];
+        id -= st[lo];
+    }
     const AP a0 = (AP)((const char *)pa + (long)ctx * kCtxArgBytes);
     const int tindex = 2 * *a0->step + tinc;
     constexpr int TY = lds_ty<R>(), HX = 64 + 4, HY = TY + 4;
@@ -624,7 +638,6 @@ stage_lds_kernel(const StageArgs<R> *pa, int nblk, int tinc)
     // 8*CGFD_TILE_GROUP consecutive tiles.  Both cut the L2 fetch traffic by
     // ~8 % on the 600x600x300 model yet run slower than the identity map.
     const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
-    const int nb = nblk, id = blockIdx.x - ctx * nblk;
 #ifndef CGFD_TILE_MAP
 #define CGFD_TILE_MAP 1
 #endif
@@ -1482,18 +1495,17 @@ __global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, int SX, int SZ, int ST>
-static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx, int tinc,
-                             int rows, int zchunks, hipStream_t s)
+static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, const int *starts, int nctx,
+                             int nblocks, int tinc, int rows, int zchunks, hipStream_t s)
 {
     if (variant == 0) {
         constexpr int ty = lds_ty<R>();
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
-        const int nblk = ntx * nty * zchunks;
-        const dim3 grid((unsigned)((long)nblk * nctx));
+        const dim3 grid((unsigned)(nblocks > 0 ? nblocks : ntx * nty * zchunks));
         if (a.hmet)
-            stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, nblk, tinc);
+            stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc);
         else
-            stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, nblk, tinc);
+            stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc);
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
         stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tinc);
@@ -1505,25 +1517,38 @@ static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs
 }
 
 template <typename R, int SX, int SZ>
-static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx, int tinc,
-                             int rows, int zchunks, hipStream_t s)
+static void launch_stage_sxz(int variant, int st, const StageArgs<R> &a, const StageArgs<R> *pa, const int *starts,
+                             int nctx, int nblocks, int tinc, int rows, int zchunks, hipStream_t s)
 {
     switch (st) {
-    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
-    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
-    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
-    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, nctx, tinc, rows, zchunks, s); break;
+    case 0: launch_stage_one<R, SX, SZ, 0>(variant, a, pa, starts, nctx, nblocks, tinc, rows, zchunks, s); break;
+    case 1: launch_stage_one<R, SX, SZ, 1>(variant, a, pa, starts, nctx, nblocks, tinc, rows, zchunks, s); break;
+    case 2: launch_stage_one<R, SX, SZ, 2>(variant, a, pa, starts, nctx, nblocks, tinc, rows, zchunks, s); break;
+    default: launch_stage_one<R, SX, SZ, 3>(variant, a, pa, starts, nctx, nblocks, tinc, rows, zchunks, s); break;
     }
 }
 
 template <typename R>
-void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int nctx,
-                  int tinc, int rows, int zchunks, hipStream_t s)
+void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, const StageArgs<R> *pa, int tinc,
+                  int rows, int zchunks, hipStream_t s)
 {
-    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
-    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
-    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
-    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, nctx, tinc, rows, zchunks, s);
+    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(variant, st, a, pa, nullptr, 1, 0, tinc, rows, zchunks, s);
+    else if (sx > 0) launch_stage_sxz<R, 1, -1>(variant, st, a, pa, nullptr, 1, 0, tinc, rows, zchunks, s);
+    else if (sz > 0) launch_stage_sxz<R, -1, 1>(variant, st, a, pa, nullptr, 1, 0, tinc, rows, zchunks, s);
+    else launch_stage_sxz<R, -1, -1>(variant, st, a, pa, nullptr, 1, 0, tinc, rows, zchunks, s);
+}
+
+template <typename R>
+void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
+                        int nblocks, int tinc, hipStream_t s)
+{
+    if (nctx < 1 || nblocks < 1) return;
+    StageArgs<R> a{}; // only the metric mode steers a batched launch
+    a.hmet = hmet;
+    if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
+    else if (sx > 0) launch_stage_sxz<R, 1, -1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
+    else if (sz > 0) launch_stage_sxz<R, -1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
+    else launch_stage_sxz<R, -1, -1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
 }
 
 template <typename R, int SPLIT>
@@ -1536,12 +1561,11 @@ static void launch_timg_split(int sx, int sz, const TimgArgs<R> *pa, dim3 grid, 
 }
 
 template <typename R>
-void launch_timg(int sx, int sz, const TimgArgs<R> &a, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s)
+void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s)
 {
     dim3 block(64, rows);
-    const long cols = (long)a.g.ni * a.g.nj * nctx;
     const int split = cols < 65536 ? 9 : (cols < 131072 ? 3 : 1); // see timg_kernel
-    dim3 grid((a.g.ni + 63) / 64, (a.g.nj + rows - 1) / rows, split * nctx);
+    dim3 grid((ni + 63) / 64, (nj + rows - 1) / rows, split * nctx);
     if (split == 9) launch_timg_split<R, 9>(sx, sz, pa, grid, block, s);
     else if (split == 3) launch_timg_split<R, 3>(sx, sz, pa, grid, block, s);
     else launch_timg_split<R, 1>(sx, sz, pa, grid, block, s);
@@ -1641,8 +1665,10 @@ void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s)
 
 #define CGFD_INSTANTIATE(R)                                                                                   \
     template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int, \
-                                   int, hipStream_t);                                                         \
-    template void launch_timg<R>(int, int, const TimgArgs<R> &, const TimgArgs<R> *, int, int, hipStream_t); \
+                                   hipStream_t);                                                              \
+    template void launch_stage_batch<R>(int, int, int, int, const StageArgs<R> *, const int *, int, int, int,   \
+                                        hipStream_t);                                                         \
+    template void launch_timg<R>(int, int, int, int, long, const TimgArgs<R> *, int, int, hipStream_t);      \
     template void launch_record_batch<R>(const RecArgs<R> *, int, int, hipStream_t);                          \
     template void launch_velpd<R>(const Geom &, const R *, const R *, const R *, R *, long, hipStream_t);     \
     template void launch_to_padded<R>(const R *, R *, const Geom &, hipStream_t);                            \

@@ -130,13 +130,13 @@ int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
 int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
 /* 1 if cgfd_run_batch runs this list as one launch per kernel (no reference
  * counterpart): 2 to 4096 distinct contexts on one device with the same
- * precision and grid dimensions (hence one padded layout; media, PML,
- * sources and receivers may differ), the default stage variant, the same
- * metric mode and the same step phase (step % 4), none timing its kernels.
- * Such a batch runs on the first context's stream, which waits for the
- * others' queued work, with its own z-chunk count (the contexts' tuning is
- * left as it is); the other streams wait for the batch.  Otherwise 0 (or
- * CGFD_EINVAL for a null list). */
+ * precision, the default stage variant, the same metric mode and the same
+ * step phase (step % 4), none timing its kernels.  Grid sizes, media, PML,
+ * sources, receivers and snapshots may differ.  Such a batch runs on the
+ * first context's stream, which waits for the others' queued work, with its
+ * own z-chunk count (the contexts' tuning is left as it is); the other
+ * streams wait for the batch.  Otherwise 0 (or CGFD_EINVAL for a null
+ * list). */
 int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few
  * z-chunk counts around the default and keep the fastest (*chosen); with

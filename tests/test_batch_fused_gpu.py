@@ -1,11 +1,11 @@
-"""Batched launches (cgfd_run_batch on a fusable list): grids of one size
-advanced by one launch per kernel and stage give, value for value, the
-traces, PGV, snapshots and final state of running each grid alone, also
-with different media, PML widths, sources and receiver counts per grid,
-with single runs interleaved between batches, and in FP64.  Lists that
-cannot share a launch (other sizes, another step phase, a repeated
-context, another stage variant, kernel timing) are reported as such and
-still run correctly on one stream per grid."""
+"""Batched launches (cgfd_run_batch on a fusable list): grids advanced by
+one launch per kernel and stage give, value for value, the traces, PGV,
+snapshots and final state of running each grid alone, with different
+sizes, media, PML widths, sources and receiver counts per grid, with
+single runs interleaved between batches, and in FP64.  Lists that cannot
+share a launch (another step phase or metric mode, a repeated context,
+another stage variant, kernel timing) are reported as such and still run
+correctly on one stream per grid."""
 import numpy as np
 import pytest
 
@@ -21,13 +21,16 @@ PMLS = ([4, 4, 3, 3, 4, 0], [2, 5, 4, 2, 3, 0], [0, 0, 0, 0, 0, 0])
 NRECV = (6, 2, 9)
 
 
-def _scenarios(nt, n=3, ni=NI):
+def _scenarios(nt, n=3, ni=NI, sizes=None, shear=True):
     scs = []
     for i in range(n):
-        sc = scenario.config_random(ni, NJ, NK, nt, PMLS[i % 3], seed=90 + i, nfoc=2 if i == 1 else 0,
-                                    nrecv=NRECV[i % 3])
-        sc["snapshots"] = np.array([[1, 1, NK, ni, NJ, 1, 1, 1, 1, 3 + i % 3, 1],
-                                    [2 + i, 2, 2, 4, 3, 5, 5, 6, 3, 1, 3]], np.int32)
+        gi, gj, gk = sizes[i] if sizes else (ni, NJ, NK)
+        sc = scenario.config_random(gi, gj, gk, nt, PMLS[i % 3], seed=90 + i, nfoc=2 if i == 1 else 0,
+                                    nrecv=NRECV[i % 3], shear=shear)
+        x0 = 2 + i % 3  # a 4 x 3 x 3 box spread over the grid, every step
+        sc["snapshots"] = np.array([[1, 1, gk, gi, gj, 1, 1, 1, 1, 3 + i % 3, 1],
+                                    [x0, 2, 2, 4, 3, 3, max(1, (gi - x0) // 3), max(1, (gj - 2) // 2),
+                                     max(1, (gk - 2) // 2), 1, 3]], np.int32)
         scs.append(sc)
     return scs
 
@@ -100,6 +103,21 @@ def test_fused_batch_with_graphs_on():
     _check_against_single(batch, scs, 14)
 
 
+SIZES = [(36, 30, 20), (70, 33, 26), (20, 17, 12), (65, 64, 30), (129, 9, 8)]
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_fused_batch_mixed_sizes(double):
+    """Grids of different sizes in one launch: each finds its workgroups by a
+    search over the batch's prefix table and keeps its own z-chunk length."""
+    scs = _scenarios(11, n=len(SIZES), sizes=SIZES)
+    batch = [_sim(sc, double) for sc in scs]
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 6)
+    cgfd3d.run_batch(batch[::-1], 5)  # another leading grid and order
+    _check_against_single(batch, scs, 11, double)
+
+
 def test_fused_batch_many_grids():
     scs = _scenarios(9, n=12)
     batch = [_sim(sc) for sc in scs]
@@ -113,7 +131,8 @@ def test_unfusable_lists_are_reported_and_still_run():
     batch = [_sim(sc) for sc in scs]
     assert not cgfd3d.batch_fusable(batch[:1])
     assert not cgfd3d.batch_fusable([batch[0], batch[1], batch[0]])
-    other = _sim(_scenarios(12, n=1, ni=NI + 4)[0])
+    other = _sim(_scenarios(12, n=1, shear=False)[0])   # constant horizontal metrics
+    assert other.metrics_specialized() and not batch[0].metrics_specialized()
     assert not cgfd3d.batch_fusable([batch[0], other])
     batch[2].set_variant(1)
     assert not cgfd3d.batch_fusable(batch)

@@ -25,6 +25,7 @@ def test_batch_equals_single_runs():
            scenario.config_random(20, 17, 9, 22, [2, 2, 2, 2, 2, 0], seed=73, shear=False),
            scenario.config_random(64, 12, 30, 24, [0, 0, 0, 0, 0, 0], seed=74)]
     batch = [_sim(sc) for sc in scs]
+    assert not cgfd3d.batch_fusable(batch)   # metric modes differ: one stream per grid
     steps = 20
     cgfd3d.run_batch(batch, steps)
     for sc, b in zip(scs, batch):

@@ -78,9 +78,11 @@ def test_graph_recaptures_after_tuning_change():
 
 
 def test_batch_with_graphs_equals_single_eager_runs():
-    # contexts of different sizes advanced by cgfd_run_batch with graph
-    # replay: 2 whole cycles, then 2 eager steps
-    scs = [scenario.config_random(n, n - 4, 16, 10, [3, 3, 3, 3, 3, 0], seed=90 + n) for n in (24, 30, 36)]
+    # contexts of different sizes and metric modes (so one stream each)
+    # advanced by cgfd_run_batch with graph replay: 2 whole cycles, then 2
+    # eager steps
+    scs = [scenario.config_random(n, n - 4, 16, 10, [3, 3, 3, 3, 3, 0], seed=90 + n, shear=n != 30)
+           for n in (24, 30, 36)]
     want = []
     for sc in scs:
         e = _sim(sc)
@@ -89,6 +91,7 @@ def test_batch_with_graphs_equals_single_eager_runs():
     sims = [_sim(sc) for sc in scs]
     for g in sims:
         g.set_graph(True)
+    assert not cgfd3d.batch_fusable(sims)
     cgfd3d.run_batch(sims, 10)
     for g, (seis, fields) in zip(sims, want):
         assert g.step == 10

@@ -87,9 +87,10 @@ def gemm(a, b, out, accumulate=False):
     return out
 
 
-def _weight_grad(xb, gyb, chunk=4096):
+def _weight_grad(xb, gyb, chunk=512):
     """x^T @ gy for tall x [R, K] and gy [R, N]: R is split into chunks (the
-    batch index of the GEMM) whose partial products are summed."""
+    batch index of the GEMM, enough of them to fill the chip) whose partial
+    products are summed."""
     R, K = xb.shape
     N = gyb.shape[1]
     full = R // chunk
@@ -106,15 +107,20 @@ def _weight_grad(xb, gyb, chunk=4096):
 
 
 class _Matmul(torch.autograd.Function):
-    """y = x @ w with x [R, K], w [K, N] (fp32 masters, bf16 on the cores)."""
+    """y = x @ w with x [R, K], w [K, N] (fp32 masters, bf16 on the cores).
+    K is zero-padded to a multiple of 8 so that x's rows stage with 16-byte
+    loads; the padding adds exact zeros and is cut from the gradients."""
 
     @staticmethod
     def forward(ctx, x, w):
-        xb = x.contiguous().to(torch.bfloat16)
-        wb = w.contiguous().to(torch.bfloat16)
+        K = x.shape[1]
+        pad = -K % 8
+        xb = torch.nn.functional.pad(x.to(torch.bfloat16), (0, pad)).contiguous()
+        wb = torch.nn.functional.pad(w.to(torch.bfloat16), (0, 0, 0, pad)).contiguous()
         y = torch.empty(x.shape[0], w.shape[1], device=x.device, dtype=torch.float32)
         gemm(xb, wb, y)
         ctx.save_for_backward(xb, wb)
+        ctx.k = K
         return y
 
     @staticmethod
@@ -123,7 +129,7 @@ class _Matmul(torch.autograd.Function):
         gyb = gy.contiguous().to(torch.bfloat16)
         gx = torch.empty(xb.shape, device=gy.device, dtype=torch.float32)
         gemm(gyb, wb.t(), gx)
-        return gx, _weight_grad(xb, gyb)
+        return gx[:, :ctx.k], _weight_grad(xb, gyb)[:ctx.k]
 
 
 class _Bmm(torch.autograd.Function):

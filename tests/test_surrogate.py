@@ -69,15 +69,30 @@ def test_library_missing_raises(monkeypatch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("M,N,K,batch,ta,tb", [(64, 64, 32, 1, False, False), (70, 33, 45, 3, False, False),
-                                               (130, 96, 200, 2, True, False), (17, 129, 64, 4, False, True),
-                                               (1000, 48, 7, 1, True, True)])
-def test_gemm_matches_numpy(M, N, K, batch, ta, tb):
+@pytest.mark.parametrize("M,N,K,batch,ta,tb,off", [
+    (64, 64, 32, 1, False, False, 0), (70, 33, 45, 3, False, False, 0),
+    (130, 96, 200, 2, True, False, 0), (17, 129, 64, 4, False, True, 0),
+    (1000, 48, 7, 1, True, True, 0),
+    # strides in multiples of 8: 16-byte staging along k (A), along n (B),
+    # along m (transposed A) and along k (transposed B), with ragged edges
+    (72, 80, 40, 2, False, False, 0), (136, 24, 48, 3, True, True, 0), (9, 200, 1000, 2, False, False, 0),
+    # the same shapes from a misaligned base: single-element staging
+    (72, 80, 40, 2, False, False, 3), (136, 24, 48, 3, True, True, 5)])
+def test_gemm_matches_numpy(M, N, K, batch, ta, tb, off):
     rng = np.random.default_rng(M + N + K)
     a = rng.standard_normal((batch, M, K)).astype(np.float32)
     b = rng.standard_normal((batch, K, N)).astype(np.float32)
-    A = _bf16(np.ascontiguousarray(a.transpose(0, 2, 1)) if ta else a).cuda()
-    B = _bf16(np.ascontiguousarray(b.transpose(0, 2, 1)) if tb else b).cuda()
+
+    def dev(x):  # on the GPU, off elements into a larger buffer
+        t = _bf16(x).cuda()
+        if not off:
+            return t
+        buf = torch.zeros(t.numel() + off, dtype=t.dtype, device="cuda")
+        buf[off:] = t.reshape(-1)
+        return buf[off:].view(t.shape)
+
+    A = dev(np.ascontiguousarray(a.transpose(0, 2, 1)) if ta else a)
+    B = dev(np.ascontiguousarray(b.transpose(0, 2, 1)) if tb else b)
     A = A.transpose(1, 2) if ta else A
     B = B.transpose(1, 2) if tb else B
     out = torch.full((batch, M, N), 0.5, device="cuda")

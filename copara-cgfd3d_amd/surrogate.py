@@ -10,9 +10,12 @@ engine's PGV capture produces.
 - **Matrix products.** Every product of the network runs on the BF16 MFMA
   kernel of libcgfd_fno.so (FP32 accumulation), in the forward pass and in
   both gradients of the backward pass: the lifting and projection layers,
-  the pointwise channel mixing and the per-mode complex spectral mixing
-  (realified as 2x2 real blocks). PyTorch supplies device memory, the FFTs,
-  GELU and the optimizer.
+  the pointwise channel mixing, the per-mode complex spectral mixing
+  (realified as 2x2 real blocks), and the truncated Fourier transforms
+  themselves: only the kept modes are ever formed, so each forward and
+  inverse transform is a product with a small constant DFT matrix on the
+  channels-last activations.  PyTorch supplies device memory, GELU and the
+  optimizer; its FFT runs only in the FP32 reference path.
 - **Gate.** Two networks trained from different seeds. Their disagreement is
   the error indicator, and scenarios above the threshold are re-run by the
   FD engine.
@@ -155,6 +158,78 @@ class _Bmm(torch.autograd.Function):
         return gx, gw
 
 
+class _LeftConst(torch.autograd.Function):
+    """y[z] = a @ x[z] for a constant matrix a [M, K] (bf16) and x [Z, K, N]:
+    the truncated DFTs of the spectral layers."""
+
+    @staticmethod
+    def forward(ctx, x, a):
+        Z, K, N = x.shape
+        xb = x.contiguous().to(torch.bfloat16)
+        y = torch.empty(Z, a.shape[0], N, device=x.device, dtype=torch.float32)
+        gemm(a.unsqueeze(0).expand(Z, -1, -1), xb, y)
+        ctx.save_for_backward(a)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        (a,) = ctx.saved_tensors
+        Z, M, N = gy.shape
+        gyb = gy.contiguous().to(torch.bfloat16)
+        gx = torch.empty(Z, a.shape[1], N, device=gy.device, dtype=torch.float32)
+        gemm(a.t().unsqueeze(0).expand(Z, -1, -1), gyb, gx)
+        return gx, None
+
+
+_DFT = {}
+
+
+def dft_matrices(H, W, m, device):
+    """The truncated transforms of an FNO layer on an H x W grid keeping
+    wavenumbers kw < m and kh in [0, m) u [H-m, H) (the kept kh indexed
+    0..2m-1), as bf16 matrices (computed in float64):
+
+    fw [2m, W]    rows (re/im, kw): the forward real DFT along w
+    gh [4m, 2H]   rows (re/im, kh index), columns (h, re/im): the forward
+                  complex DFT along h, realified
+    gi [2H, 4m]   rows (h, re/im), columns (kh index, re/im): the inverse
+                  complex DFT along h, realified
+    fi [W, 2m]    columns (re/im, kw): the inverse real DFT along w with the
+                  one-sided weights (1 at kw = 0, else 2) and 1/(H W)
+
+    These reproduce numpy's (and torch's) rfft2 / irfft2 conventions: e^-i
+    forward, e^+i inverse, no forward scaling, and the imaginary part of the
+    kw = 0 column ignored by the real inverse."""
+    key = (H, W, m, str(device))
+    if key not in _DFT:
+        _DFT[key] = tuple(torch.from_numpy(a).to(device=device, dtype=torch.bfloat16).contiguous()
+                          for a in dft_arrays(H, W, m))
+    return _DFT[key]
+
+
+def dft_arrays(H, W, m):
+    """The float64 arrays behind dft_matrices (same shapes and layouts)."""
+    if not (2 * m <= H and m <= W // 2):
+        raise SurrogateError(f"{m} modes do not fit an {H} x {W} grid")
+    kw = np.arange(m)[:, None]
+    w = np.arange(W)[None, :]
+    phi = 2 * np.pi * kw * w / W                                   # [m, W]
+    fw = np.concatenate([np.cos(phi), -np.sin(phi)])               # [2m, W]
+    kh = np.concatenate([np.arange(m), np.arange(H - m, H)])[:, None]
+    h = np.arange(H)[None, :]
+    th = 2 * np.pi * kh * h / H                                    # [2m, H]
+    c, sn = np.cos(th), np.sin(th)
+    gh = np.empty((2, 2 * m, H, 2))
+    gh[0, :, :, 0], gh[0, :, :, 1] = c, sn                         # re = cos*re + sin*im
+    gh[1, :, :, 0], gh[1, :, :, 1] = -sn, c                        # im = cos*im - sin*re
+    gi = np.empty((H, 2, 2 * m, 2))
+    gi[:, 0, :, 0], gi[:, 0, :, 1] = c.T, -sn.T                    # re = cos*re - sin*im
+    gi[:, 1, :, 0], gi[:, 1, :, 1] = sn.T, c.T                     # im = sin*re + cos*im
+    wt = np.where(np.arange(m) == 0, 1.0, 2.0)[None, :] / (H * W)
+    fi = np.concatenate([wt * np.cos(phi.T), -wt * np.sin(phi.T)], axis=1)   # [W, 2m]
+    return tuple(np.ascontiguousarray(a.reshape(r, -1)) for a, r in ((fw, 2 * m), (gh, 4 * m), (gi, 2 * H), (fi, W)))
+
+
 def matmul(x, w, backend):
     return _Matmul.apply(x, w) if backend == "mfma" else x @ w
 
@@ -194,6 +269,8 @@ class FNO2d(torch.nn.Module):
         self.p2_w, self.p2_b = lin(proj, cout), torch.nn.Parameter(torch.zeros(cout))
 
     def _spectral(self, v, l, backend):
+        if backend == "mfma":
+            return self._spectral_dft(v, l)
         B, H, W, C = v.shape
         m = self.modes
         ft = torch.fft.rfft2(v, dim=(1, 2))                                   # [B, H, W/2+1, C]
@@ -208,6 +285,23 @@ class FNO2d(torch.nn.Module):
         out[:, :m, :m] = y[:, :m]
         out[:, H - m:, :m] = y[:, m:]
         return torch.fft.irfft2(out, s=(H, W), dim=(1, 2))
+
+    def _spectral_dft(self, v, l):
+        """The spectral layer with the truncated transforms as products with
+        the dft_matrices, on channels-last activations: no FFT and no
+        full-size transposes, only the kept modes are permuted."""
+        B, H, W, C = v.shape
+        m = self.modes
+        fw, gh, gi, fi = dft_matrices(H, W, m, v.device)
+        y1 = _LeftConst.apply(v.reshape(B * H, W, C), fw)                    # [(b,h), (ri,kw), c]
+        y2 = _LeftConst.apply(y1.reshape(B, 2 * H, m * C), gh)               # [b, (ri,kh), (kw,c)]
+        x = y2.reshape(B, 2, 2 * m, m, C).permute(2, 3, 0, 1, 4).reshape(2 * m * m, B, 2 * C)
+        wre, wim = self.spec_re[l], self.spec_im[l]
+        wr = torch.cat([torch.cat([wre, wim], 2), torch.cat([-wim, wre], 2)], 1)   # [P, 2C, 2C]
+        yr = bmm(x, wr, "mfma")                                               # [(kh,kw), b, (ri,c)]
+        z = yr.reshape(2 * m, m, B, 2, C).permute(2, 0, 3, 1, 4).reshape(B, 4 * m, m * C)
+        z1 = _LeftConst.apply(z, gi)                                          # [b, (h,ri), (kw,c)]
+        return _LeftConst.apply(z1.reshape(B * H, 2 * m, C), fi).reshape(B, H, W, C)
 
     def forward(self, x, backend="mfma"):
         B, H, W, _ = x.shape

@@ -61,6 +61,33 @@ def test_dihedral_transforms_are_orthogonal_similarities():
     assert len(images) == 8
 
 
+@pytest.mark.parametrize("B,H,W,C,m", [(2, 24, 20, 5, 6), (3, 80, 80, 4, 16), (1, 17, 30, 3, 8)])
+def test_truncated_dft_matrices_match_fft(B, H, W, C, m):
+    """The spectral layer's transform matrices, applied in float64 in the
+    layouts the MFMA path uses, give torch.fft's kept modes and, after a
+    random complex mixing, its irfft2 of them."""
+    torch.manual_seed(B + H + W)
+    v = torch.randn(B, H, W, C, dtype=torch.float64)
+    fw, gh, gi, fi = [torch.from_numpy(a) for a in surrogate.dft_arrays(H, W, m)]
+    left = lambda a, x: torch.einsum("mk,zkn->zmn", a, x)
+    y1 = left(fw, v.reshape(B * H, W, C))
+    y2 = left(gh, y1.reshape(B, 2 * H, m * C))
+    x = y2.reshape(B, 2, 2 * m, m, C).permute(2, 3, 0, 1, 4).reshape(2 * m * m, B, 2 * C)
+    ft = torch.fft.rfft2(v, dim=(1, 2))
+    sel = torch.cat([ft[:, :m, :m], ft[:, H - m:, :m]], dim=1).permute(1, 2, 0, 3).reshape(2 * m * m, B, C)
+    want = torch.cat([sel.real, sel.imag], dim=2)
+    assert float((x - want).abs().max()) <= 1e-12 * float(want.abs().max())
+    wc = torch.randn(2 * m * m, C, C, dtype=torch.complex128)
+    wr = torch.cat([torch.cat([wc.real, wc.imag], 2), torch.cat([-wc.imag, wc.real], 2)], 1)
+    z = torch.bmm(x, wr).reshape(2 * m, m, B, 2, C).permute(2, 0, 3, 1, 4).reshape(B, 4 * m, m * C)
+    out = left(fi, left(gi, z).reshape(B * H, 2 * m, C)).reshape(B, H, W, C)
+    y = torch.einsum("pbc,pcd->pbd", sel, wc).reshape(2 * m, m, B, C).permute(2, 0, 1, 3)
+    o = torch.zeros(B, H, W // 2 + 1, C, dtype=torch.complex128)
+    o[:, :m, :m], o[:, H - m:, :m] = y[:, :m], y[:, m:]
+    ref = torch.fft.irfft2(o, s=(H, W), dim=(1, 2))
+    assert float((out - ref).abs().max()) <= 1e-12 * float(ref.abs().max())
+
+
 def test_library_missing_raises(monkeypatch):
     monkeypatch.setattr(surrogate, "_LIB", None)
     monkeypatch.setenv("CGFD_FNO_LIB", "/nonexistent/libcgfd_fno.so")

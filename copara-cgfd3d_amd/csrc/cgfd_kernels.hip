@@ -1205,27 +1205,31 @@ template <typename R, int SX, int SZ, int SPLIT>
 __global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> *__restrict__ pa)
 {
     // blockIdx.z = grid * SPLIT + part: grid picks the argument block of a
-    // batch (kCtxArgBytes apart), part the split of the nine row equations
+    // batch (kCtxArgBytes apart), part the split of the nine row equations.
+    // Fields are read one by one through the constant address space, so they
+    // come in by scalar loads and the field pointers keep their global
+    // address space (global rather than flat memory instructions).
+    typedef const __attribute__((address_space(4))) TimgArgs<R> *TP;
     const int ctx = blockIdx.z / SPLIT, bz = blockIdx.z - ctx * SPLIT;
-    const TimgArgs<R> &a = *(const TimgArgs<R> *)((const char *)pa + (long)ctx * kCtxArgBytes);
-    const Geom g = a.g;
+    const TP a = (TP)((const char *)pa + (long)ctx * kCtxArgBytes);
+    const Geom g = geom_of(a);
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= g.ni || y >= g.nj) return;
     const long fs = g.fs, row = g.nxp, pl = g.plane;
-    const R *pmx = a.pml;
-    const R *pmy = a.pml + 3 * g.ni;
+    const R *pmx = a->pml;
+    const R *pmy = a->pml + 3 * g.ni;
     const R APDx = pmx[x], Bx = pmx[g.ni + x], DBx = pmx[2 * g.ni + x];
     const R APDy = pmy[y], By = pmy[g.nj + y], DBy = pmy[2 * g.nj + y];
-    const bool inx = x < a.nabs[0] || x >= g.ni - a.nabs[1];
-    const int ix = x < a.nabs[0] ? x : x - (g.ni - a.nabs[1]) + a.nabs[0];
-    const bool iny = y < a.nabs[2] || y >= g.nj - a.nabs[3];
-    const int iy = y < a.nabs[2] ? y : y - (g.nj - a.nabs[3]) + a.nabs[2];
-    const double hx = SX > 0 ? a.h : -a.h;
-    const double hz = SZ > 0 ? a.h : -a.h;
+    const bool inx = x < a->nabs[0] || x >= g.ni - a->nabs[1];
+    const int ix = x < a->nabs[0] ? x : x - (g.ni - a->nabs[1]) + a->nabs[0];
+    const bool iny = y < a->nabs[2] || y >= g.nj - a->nabs[3];
+    const int iy = y < a->nabs[2] ? y : y - (g.nj - a->nabs[3]) + a->nabs[2];
+    const double hx = SX > 0 ? a->h : -a->h;
+    const double hz = SZ > 0 ? a->h : -a->h;
     // momentum rows: (Txx,Txy,Txz) -> Vx, (Txy,Tyy,Tyz) -> Vy, (Txz,Tyz,Tzz) -> Vz
     constexpr int trow[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
-    const R *jac = a.met + JAC * fs;
+    const R *jac = a->met + JAC * fs;
 
     // The 3 top rows x 3 momentum equations are independent.  Small grids
     // split them over blockIdx.z (SPLIT = 9: one pair per block; 3: one row
@@ -1239,29 +1243,29 @@ __global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> *__restrict
         if (SPLIT > 1 && r != rz) continue;
         const int z = g.nk - 3 + r;
         const long G = gidx(g, x, y, z);
-        const R rhojac = (R)(1.0 / (double)a.med[3 * fs + G] / (double)jac[G]);
+        const R rhojac = (R)(1.0 / (double)a->med[3 * fs + G] / (double)jac[G]);
         const long tcell = ((long)r * g.nj + y) * row + x;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             if (SPLIT == 9 && m != mz) continue;
-            const R *T0 = a.win + trow[m][0] * fs;
-            const R *T1 = a.win + trow[m][1] * fs;
-            const R *T2 = a.win + trow[m][2] * fs;
+            const R *T0 = a->win + trow[m][0] * fs;
+            const R *T1 = a->win + trow[m][1] * fs;
+            const R *T2 = a->win + trow[m][2] * fs;
             R vx[7], vy[7], vz[7];
 #pragma unroll
             for (int i = 0; i < 7; ++i) {
                 vx[i] = vy[i] = 0;
                 if ((SX > 0 && i >= 2) || (SX < 0 && i <= 4)) {
                     long q = G + (i - 3);
-                    vx[i] = jac[q] * (a.met[XIX * fs + q] * T0[q] + a.met[XIY * fs + q] * T1[q] +
-                                      a.met[XIZ * fs + q] * T2[q]);
+                    vx[i] = jac[q] * (a->met[XIX * fs + q] * T0[q] + a->met[XIY * fs + q] * T1[q] +
+                                      a->met[XIZ * fs + q] * T2[q]);
                     q = G + (i - 3) * row;
-                    vy[i] = jac[q] * (a.met[ETX * fs + q] * T0[q] + a.met[ETY * fs + q] * T1[q] +
-                                      a.met[ETZ * fs + q] * T2[q]);
+                    vy[i] = jac[q] * (a->met[ETX * fs + q] * T0[q] + a->met[ETY * fs + q] * T1[q] +
+                                      a->met[ETZ * fs + q] * T2[q]);
                 }
                 const long q = G + (i - 3) * pl;
-                vz[i] = jac[q] * (a.met[ZTX * fs + q] * T0[q] + a.met[ZTY * fs + q] * T1[q] +
-                                  a.met[ZTZ * fs + q] * T2[q]);
+                vz[i] = jac[q] * (a->met[ZTX * fs + q] * T0[q] + a->met[ZTY * fs + q] * T1[q] +
+                                  a->met[ZTZ * fs + q] * T2[q]);
             }
             // odd images above the surface, zero traction on it
             const int s = 3 + (2 - r);
@@ -1275,16 +1279,16 @@ __global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> *__restrict
             const R DzTz = (R)((double)rhojac * (drp_sum_strf<R, SZ>(vz) / hz));
             R hv = (Bx == (R)1 && By == (R)1) ? DxTx + DyTy + DzTz : DxTx / Bx + DyTy / By + DzTz;
             if (inx) {
-                const R av = a.ain[0][m * a.astr[0] + ((long)z * g.nj + y) * a.npml[0] + ix];
+                const R av = a->ain[0][m * a->astr[0] + ((long)z * g.nj + y) * a->npml[0] + ix];
                 hv -= av / Bx;
-                a.timg[(3 + m) * a.tstr + tcell] = DBx * DxTx - APDx * av;
+                a->timg[(3 + m) * a->tstr + tcell] = DBx * DxTx - APDx * av;
             }
             if (iny) {
-                const R av = a.ain[1][m * a.astr[1] + ((long)z * a.npml[1] + iy) * row + x];
+                const R av = a->ain[1][m * a->astr[1] + ((long)z * a->npml[1] + iy) * row + x];
                 hv -= av / By;
-                a.timg[(6 + m) * a.tstr + tcell] = DBy * DyTy - APDy * av;
+                a->timg[(6 + m) * a->tstr + tcell] = DBy * DyTy - APDy * av;
             }
-            a.timg[m * a.tstr + tcell] = hv;
+            a->timg[m * a->tstr + tcell] = hv;
         }
     }
 }
@@ -1631,13 +1635,14 @@ void launch_step_advance(int *step, hipStream_t s) { step_advance_kernel<<<1, 64
 // kCtxArgBytes from the previous grid's
 template <typename R> __global__ void record_batch_kernel(const RecArgs<R> *__restrict__ pa)
 {
-    const RecArgs<R> &a = *(const RecArgs<R> *)((const char *)pa + (long)blockIdx.y * kCtxArgBytes);
+    typedef const __attribute__((address_space(4))) RecArgs<R> *RP;
+    const RP a = (RP)((const char *)pa + (long)blockIdx.y * kCtxArgBytes);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nrecv * NF) return;
-    const int t = *a.step;
-    if (t >= a.nt) return;
+    if (i >= a->nrecv * NF) return;
+    const int t = *a->step;
+    if (t >= a->nt) return;
     const int r = i / NF, c = i % NF;
-    a.seis[((long)c * a.nrecv + r) * a.nt + t] = a.fw[c * a.fs + a.rlin[r]];
+    a->seis[((long)c * a->nrecv + r) * a->nt + t] = a->fw[c * a->fs + a->rlin[r]];
 }
 
 template <typename R> void launch_record_batch(const RecArgs<R> *pa, int nctx, int max_nrecv, hipStream_t s)

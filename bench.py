@@ -74,6 +74,8 @@ def batch_bench(args):
     import cgfd3d
     import scenario
 
+    if args.full:
+        return batch_full_bench(args)
     nt = args.warmup + args.steps
     t0 = time.perf_counter()
     scs = scenario.config_scenario_batch(args.batch, nt_range=(nt, nt), seed=11)
@@ -120,6 +122,81 @@ def batch_bench(args):
                    "scenario_steps_per_s_batched": round(len(scs) * args.steps / t_bat, 2),
                    "scenario_steps_per_s_sequential": round(len(scs) * args.steps / t_seq, 2),
                    "speedup_vs_sequential": round(t_seq / t_bat, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def batch_full_bench(args):
+    """The uneven-scenario batch run to completion: `--batch` hill scenarios
+    with their own sizes (96-192 x 96-192 x 64-128 cells) and lengths
+    (800-2500 steps), 32 stations and PGV each.  Batched: every call of
+    cgfd_run_batch advances all unfinished scenarios (one launch per kernel
+    and stage) up to the next scenario's end.  Sequential: each scenario
+    alone, to its end.  The value is whole-job cell-stages per second of the
+    batched run; scenarios/s of both runs are in config."""
+    import numpy as np
+
+    import cgfd3d
+    import scenario
+
+    t0 = time.perf_counter()
+    scs = scenario.config_scenario_batch(args.batch, seed=11)
+    log(f"{len(scs)} scenarios built in {time.perf_counter() - t0:.1f} s")
+    sims = []
+    for sc in scs:
+        g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+        g.setup(sc)
+        sims.append(g)
+    nts = [sc["nt"] for sc in scs]
+    work = sum(sc["ni"] * sc["nj"] * sc["nk"] * 4 * sc["nt"] for sc in scs)
+
+    def reset_and_warm():
+        for g in sims:
+            g.init()
+        cgfd3d.run_batch(sims, 2)   # first launches of every kernel instance
+        for g in sims:
+            g.init()
+            g.sync()
+
+    reset_and_warm()
+    t = time.perf_counter()
+    for g, nt in zip(sims, nts):
+        g.run(nt)
+        g.sync()
+    t_seq = time.perf_counter() - t
+    log(f"sequential: {t_seq:.1f} s")
+    pgv_seq = [g.get_pgv() for g in sims]
+
+    reset_and_warm()
+    t = time.perf_counter()
+    done, calls = 0, 0
+    active = list(range(len(sims)))
+    while active:
+        n = min(nts[i] - done for i in active)
+        cgfd3d.run_batch([sims[i] for i in active], n)
+        calls += 1
+        done += n
+        active = [i for i in active if nts[i] > done]
+    for g in sims:
+        g.sync()
+    t_bat = time.perf_counter() - t
+    log(f"batched: {t_bat:.1f} s in {calls} calls")
+    same = all(np.array_equal(g.get_pgv(), p) for g, p in zip(sims, pgv_seq))
+    for g in sims:
+        g.close()
+    line = {
+        "metric": "Gcell-RKstages/s", "value": round(work / t_bat / 1e9, 4), "unit": "Gcell-RKstages/s",
+        "n_gpus": 1, "steps": int(max(nts)), "warmup": 2, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch-{len(scs)}-hill-full", "steps_per_scenario": [int(n) for n in nts],
+                   "cells_total": int(sum(sc["ni"] * sc["nj"] * sc["nk"] for sc in scs)),
+                   "batch_calls": calls,
+                   "scenarios_per_s_batched": round(len(scs) / t_bat, 4),
+                   "scenarios_per_s_sequential": round(len(scs) / t_seq, 4),
+                   "speedup_vs_sequential": round(t_seq / t_bat, 3),
+                   "seconds_batched": round(t_bat, 2), "seconds_sequential": round(t_seq, 2),
+                   "pgv_equal_to_sequential": bool(same)},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -312,6 +389,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--workload", default="vonkarman", choices=["vonkarman", "hill", "flat", "batch", "surrogate"])
     ap.add_argument("--batch", type=int, default=16, help="scenarios of the batch workload")
+    ap.add_argument("--full", action="store_true",
+                    help="batch workload: run every scenario to its own end (800-2500 steps) instead of --steps")
     ap.add_argument("--train", type=int, default=512, help="training scenarios of the surrogate workload")
     ap.add_argument("--calib", type=int, default=32, help="gate-calibration scenarios of the surrogate workload")
     ap.add_argument("--test", type=int, default=64, help="test scenarios of the surrogate workload")

@@ -103,6 +103,21 @@ def test_fused_batch_with_graphs_on():
     _check_against_single(batch, scs, 14)
 
 
+def test_graph_replay_around_fused_batches():
+    """Grids that replay captured cycles alone, then join a fused batch, then
+    replay again: the captured launches read the step from the device
+    counter that the batch advanced."""
+    scs = _scenarios(20)
+    batch = [_sim(sc) for sc in scs]
+    for b in batch:
+        b.set_graph(True)
+        b.run(4)          # captures and replays the phase-0 cycle
+    cgfd3d.run_batch(batch, 8)
+    for b in batch:
+        b.run(8)          # replays again, two cycles
+    _check_against_single(batch, scs, 20)
+
+
 SIZES = [(36, 30, 20), (70, 33, 26), (20, 17, 12), (65, 64, 30), (129, 9, 8)]
 
 

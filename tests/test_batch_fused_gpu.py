@@ -22,10 +22,12 @@ NRECV = (6, 2, 9)
 
 
 def _scenarios(nt, n=3, ni=NI, sizes=None, shear=True):
+    """n random scenarios; nt is one step count for all or one per scenario."""
     scs = []
     for i in range(n):
         gi, gj, gk = sizes[i] if sizes else (ni, NJ, NK)
-        sc = scenario.config_random(gi, gj, gk, nt, PMLS[i % 3], seed=90 + i, nfoc=2 if i == 1 else 0,
+        nti = nt[i] if isinstance(nt, (list, tuple)) else nt
+        sc = scenario.config_random(gi, gj, gk, nti, PMLS[i % 3], seed=90 + i, nfoc=2 if i == 1 else 0,
                                     nrecv=NRECV[i % 3], shear=shear)
         x0 = 2 + i % 3  # a 4 x 3 x 3 box spread over the grid, every step
         sc["snapshots"] = np.array([[1, 1, gk, gi, gj, 1, 1, 1, 1, 3 + i % 3, 1],
@@ -131,6 +133,28 @@ def test_fused_batch_mixed_sizes(double):
     cgfd3d.run_batch(batch, 6)
     cgfd3d.run_batch(batch[::-1], 5)  # another leading grid and order
     _check_against_single(batch, scs, 11, double)
+
+
+def test_fused_batch_scenarios_of_different_lengths():
+    """Scenarios with their own step counts: each call advances the
+    unfinished ones to the next one's end, and finished ones leave the list
+    (so later calls fuse a different set of grids)."""
+    nts = [7, 15, 11, 15]
+    scs = _scenarios(nts, n=4, sizes=SIZES[:4])
+    batch = [_sim(sc) for sc in scs]
+    done, active = 0, list(range(len(batch)))
+    while active:
+        n = min(nts[i] - done for i in active)
+        cgfd3d.run_batch([batch[i] for i in active], n)
+        done += n
+        active = [i for i in active if nts[i] > done]
+    for b, sc, nt in zip(batch, scs, nts):
+        s = _sim(sc)
+        s.run(nt)
+        assert b.step == nt
+        want, got = _outputs(s, sc), _outputs(b, sc)
+        for k in want:
+            assert_same(got[k], want[k], f"{sc['name']} {k}")
 
 
 def test_fused_batch_many_grids():

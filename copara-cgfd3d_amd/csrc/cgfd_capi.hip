@@ -1088,7 +1088,11 @@ int cgfd_set_graph(cgfd_ctx *c, int on)
 // precision, the LDS stage variant with the same metric mode, and the step
 // phase currT % 4 (which fixes the forward/backward pattern of every
 // stage), and no context is timing its kernels.  Grid sizes may differ.
-static long batch_tiles(const cgfd_ctx *c) { return (long)((c->g.ni + 63) / 64) * ((c->g.nj + 3) / 4); }
+static long batch_tiles(const cgfd_ctx *c)
+{
+    const int ty = stage_tile_rows(c->prec == CGFD_FP64);
+    return (long)((c->g.ni + 63) / 64) * ((c->g.nj + ty - 1) / ty);
+}
 
 static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 {
@@ -1111,7 +1115,8 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 
 // Workgroups of a batched stage launch: every grid sweeps its columns in the
 // same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS workgroups
-// of 64x4 tiles over all grids and clamped to 2 planes per chunk per grid.
+// (tiles of 64 columns x stage_tile_rows rows) over all grids and clamped to
+// 2 planes per chunk per grid.
 // zlen[c] is grid c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048

@@ -91,6 +91,8 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 // argument block lies c * kCtxArgBytes after pa and it owns workgroups
 // starts[c] .. starts[c+1]-1 (starts in device memory, starts[nctx] =
 // nblocks); all grids share the step phase and the metric mode hmet.
+// rows of a stage-kernel tile (64 columns wide) in this build
+int stage_tile_rows(bool fp64);
 template <typename R>
 void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
                         int nblocks, int tinc, hipStream_t s);

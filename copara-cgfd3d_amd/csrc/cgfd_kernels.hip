@@ -1631,6 +1631,8 @@ __global__ void step_advance_kernel(int *step)
 
 void launch_step_advance(int *step, hipStream_t s) { step_advance_kernel<<<1, 64, 0, s>>>(step); }
 
+int stage_tile_rows(bool fp64) { return fp64 ? lds_ty<double>() : lds_ty<float>(); }
+
 // batched trace gathers: blockIdx.y = grid, whose RecArgs block is
 // kCtxArgBytes from the previous grid's
 template <typename R> __global__ void record_batch_kernel(const RecArgs<R> *__restrict__ pa)

@@ -941,7 +941,7 @@ template <typename R> static int run_stage(cgfd_ctx *c)
     }
     const StageArgs<R> sa = make_stage_args<R>(c, k);
     const StageArgs<R> *pa = (const StageArgs<R> *)(c->args_d + k * kStageArgsBytes);
-    const TimgArgs<R> *pt = (const TimgArgs<R> *)(c->args_d + kTimgArgsOffset + k * kTimgArgsBytes);
+    const TimgArgs<R> ta = make_timg_args<R>(c, k);
 
     // profiling: events bracket each launch on the context's stream
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -949,7 +949,7 @@ template <typename R> static int run_stage(cgfd_ctx *c)
         for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
-    launch_timg<R>(sx, sz, c->g.ni, c->g.nj, (long)c->g.ni * c->g.nj, pt, 1, c->rows, c->stream);
+    launch_timg<R>(sx, sz, c->g.ni, c->g.nj, (long)c->g.ni * c->g.nj, ta, nullptr, 1, c->rows, c->stream);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
@@ -1190,8 +1190,8 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             int sx, sz;
             fb_signs(c0->currT, k, &sx, &sz);
             const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
-            launch_timg<R>(sx, sz, ni, nj, cols, (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes),
-                           n, 4, c0->stream);
+            launch_timg<R>(sx, sz, ni, nj, cols, TimgArgs<R>{},
+                           (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n, 4, c0->stream);
             launch_stage_batch<R>(sx, sz, k, hmet, (const StageArgs<R> *)(img_d + k * kStageArgsBytes), starts_d, n,
                                   starts[n], tinc, c0->stream);
         }

@@ -96,10 +96,12 @@ int stage_tile_rows(bool fp64);
 template <typename R>
 void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
                         int nblocks, int tinc, hipStream_t s);
-// traction image of nctx grids (blocks at kCtxArgBytes strides from pa);
+// traction image: of one grid with its arguments av (pa null, nctx 1), or of
+// nctx grids whose blocks lie at kCtxArgBytes strides from pa (av unused);
 // ni x nj covers every grid's surface, cols is their total column count
 template <typename R>
-void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s);
+void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> &av, const TimgArgs<R> *pa, int nctx,
+                 int rows, hipStream_t s);
 template <typename R>
 void launch_velpd(const Geom &g, const R *met, const R *med, const R *pml, R *cov, long cstr, hipStream_t s);
 template <typename R>

@@ -1201,17 +1201,18 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
 // ---------------------------------------------------------------------------
 // traction image, top three rows (CalTIMG, calculate.cu:5428-5666)
 // ---------------------------------------------------------------------------
-template <typename R, int SX, int SZ, int SPLIT>
-__global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> *__restrict__ pa)
+template <typename R, int SX, int SZ, int SPLIT, int BATCH>
+__global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> av, const TimgArgs<R> *__restrict__ pa)
 {
-    // blockIdx.z = grid * SPLIT + part: grid picks the argument block of a
-    // batch (kCtxArgBytes apart), part the split of the nine row equations.
-    // Fields are read one by one through the constant address space, so they
-    // come in by scalar loads and the field pointers keep their global
-    // address space (global rather than flat memory instructions).
+    // One grid (BATCH = 0): the arguments come by value (kernel-argument
+    // segment).  A batch reads pa instead: blockIdx.z = grid * SPLIT + part,
+    // and grid picks the argument block (kCtxArgBytes apart).  part is the split of
+    // the nine row equations.  Fields are read one by one through the
+    // constant address space, so they come in by scalar loads and the field
+    // pointers keep their global address space.
     typedef const __attribute__((address_space(4))) TimgArgs<R> *TP;
     const int ctx = blockIdx.z / SPLIT, bz = blockIdx.z - ctx * SPLIT;
-    const TP a = (TP)((const char *)pa + (long)ctx * kCtxArgBytes);
+    const TP a = BATCH ? (TP)((const char *)pa + (long)ctx * kCtxArgBytes) : (TP)&av;
     const Geom g = geom_of(a);
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1556,23 +1557,32 @@ void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa
 }
 
 template <typename R, int SPLIT>
-static void launch_timg_split(int sx, int sz, const TimgArgs<R> *pa, dim3 grid, dim3 block, hipStream_t s)
+static void launch_timg_split(int sx, int sz, const TimgArgs<R> &av, const TimgArgs<R> *pa, dim3 grid, dim3 block,
+                              hipStream_t s)
 {
-    if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT><<<grid, block, 0, s>>>(pa);
-    else if (sx > 0) timg_kernel<R, 1, -1, SPLIT><<<grid, block, 0, s>>>(pa);
-    else if (sz > 0) timg_kernel<R, -1, 1, SPLIT><<<grid, block, 0, s>>>(pa);
-    else timg_kernel<R, -1, -1, SPLIT><<<grid, block, 0, s>>>(pa);
+    if (pa) {
+        if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT, 1><<<grid, block, 0, s>>>(av, pa);
+        else if (sx > 0) timg_kernel<R, 1, -1, SPLIT, 1><<<grid, block, 0, s>>>(av, pa);
+        else if (sz > 0) timg_kernel<R, -1, 1, SPLIT, 1><<<grid, block, 0, s>>>(av, pa);
+        else timg_kernel<R, -1, -1, SPLIT, 1><<<grid, block, 0, s>>>(av, pa);
+    } else {
+        if (sx > 0 && sz > 0) timg_kernel<R, 1, 1, SPLIT, 0><<<grid, block, 0, s>>>(av, pa);
+        else if (sx > 0) timg_kernel<R, 1, -1, SPLIT, 0><<<grid, block, 0, s>>>(av, pa);
+        else if (sz > 0) timg_kernel<R, -1, 1, SPLIT, 0><<<grid, block, 0, s>>>(av, pa);
+        else timg_kernel<R, -1, -1, SPLIT, 0><<<grid, block, 0, s>>>(av, pa);
+    }
 }
 
 template <typename R>
-void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> *pa, int nctx, int rows, hipStream_t s)
+void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> &av, const TimgArgs<R> *pa, int nctx,
+                 int rows, hipStream_t s)
 {
     dim3 block(64, rows);
     const int split = cols < 65536 ? 9 : (cols < 131072 ? 3 : 1); // see timg_kernel
     dim3 grid((ni + 63) / 64, (nj + rows - 1) / rows, split * nctx);
-    if (split == 9) launch_timg_split<R, 9>(sx, sz, pa, grid, block, s);
-    else if (split == 3) launch_timg_split<R, 3>(sx, sz, pa, grid, block, s);
-    else launch_timg_split<R, 1>(sx, sz, pa, grid, block, s);
+    if (split == 9) launch_timg_split<R, 9>(sx, sz, av, pa, grid, block, s);
+    else if (split == 3) launch_timg_split<R, 3>(sx, sz, av, pa, grid, block, s);
+    else launch_timg_split<R, 1>(sx, sz, av, pa, grid, block, s);
 }
 
 template <typename R>
@@ -1675,7 +1685,8 @@ void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s)
                                    hipStream_t);                                                              \
     template void launch_stage_batch<R>(int, int, int, int, const StageArgs<R> *, const int *, int, int, int,   \
                                         hipStream_t);                                                         \
-    template void launch_timg<R>(int, int, int, int, long, const TimgArgs<R> *, int, int, hipStream_t);      \
+    template void launch_timg<R>(int, int, int, int, long, const TimgArgs<R> &, const TimgArgs<R> *, int, int, \
+                                  hipStream_t);                                                               \
     template void launch_record_batch<R>(const RecArgs<R> *, int, int, hipStream_t);                          \
     template void launch_velpd<R>(const Geom &, const R *, const R *, const R *, R *, long, hipStream_t);     \
     template void launch_to_padded<R>(const R *, R *, const Geom &, hipStream_t);                            \

@@ -512,17 +512,19 @@ class Surrogate:
             sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, epochs)
             g = torch.Generator().manual_seed(100 + k)
             for ep in range(epochs):
-                perm = torch.randperm(len(x), generator=g)
-                tot = 0.0
+                # the epoch's order on the device and its loss read once per
+                # epoch, so the steps queue without waiting for each other
+                perm = torch.randperm(len(x), generator=g).to(self.device)
+                tot = torch.zeros((), device=self.device)
                 for b0 in range(0, len(x), batch):
-                    idx = perm[b0:b0 + batch].to(self.device)
+                    idx = perm[b0:b0 + batch]
                     loss = torch.mean((net(x[idx]) - y[idx]) ** 2)
                     opt.zero_grad()
                     loss.backward()
                     opt.step()
-                    tot += loss.item() * len(idx)
+                    tot += loss.detach() * len(idx)
                 sched.step()
-                losses.append(tot / len(x))
+                losses.append(float(tot) / len(x))
                 if log and (ep % 10 == 0 or ep == epochs - 1):
                     log(f"net {k} epoch {ep}: loss {losses[-1]:.4f}")
         return losses

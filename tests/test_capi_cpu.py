@@ -60,6 +60,19 @@ def test_create_rejects_bad_dims_without_touching_the_gpu():
     assert not h.value
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libcgfd3d.so not built")
+def test_batch_entry_points_validate_without_the_gpu():
+    import cgfd3d
+    lib = cgfd3d.load_library()
+    assert lib.cgfd_batch_fusable(None, 2) == -1                # no list
+    assert lib.cgfd_run_batch(None, 2, 1) == -1
+    empty = (ctypes.c_void_p * 2)(None, None)
+    assert lib.cgfd_batch_fusable(empty, 2) == -1               # null contexts
+    assert lib.cgfd_run_batch(empty, 2, 1) == -1
+    assert lib.cgfd_batch_fusable(empty, 0) == 0                # an empty list is not fusable
+    assert lib.cgfd_batch_fusable(empty, -1) == -1
+
+
 FNO_HEADER = os.path.join(ROOT, "include", "cgfd_fno.h")
 FNO_LIB = os.path.join(ROOT, "copara-cgfd3d_amd", "libcgfd_fno.so")
 

@@ -88,6 +88,15 @@ def test_truncated_dft_matrices_match_fft(B, H, W, C, m):
     assert float((out - ref).abs().max()) <= 1e-12 * float(ref.abs().max())
 
 
+def test_dft_matrices_reject_modes_that_do_not_fit():
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.dft_arrays(20, 40, 11)     # 2m > H
+    with pytest.raises(surrogate.SurrogateError):
+        surrogate.dft_arrays(40, 20, 11)     # m > W/2
+    fw, gh, gi, fi = surrogate.dft_arrays(40, 20, 10)
+    assert fw.shape == (20, 20) and gh.shape == (40, 80) and gi.shape == (80, 40) and fi.shape == (20, 20)
+
+
 def test_library_missing_raises(monkeypatch):
     monkeypatch.setattr(surrogate, "_LIB", None)
     monkeypatch.setenv("CGFD_FNO_LIB", "/nonexistent/libcgfd_fno.so")

@@ -1,5 +1,6 @@
 """Fourier-neural-operator surrogate for surface peak ground velocity, with
-the finite-difference engine as its data source, fallback and judge.
+the finite-difference engine as its data source, its fallback and the
+reference its errors are measured against.
 
 For a fixed topography and medium, the surrogate maps a source
 (x, y, depth, strike, dip, rake, log10 M0) to the three surface PGV maps the

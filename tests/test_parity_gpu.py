@@ -2,8 +2,8 @@
 
 Every comparison is bit-for-bit (value equality): the engine reproduces the
 rounding of each reference operation (FP64 stencil accumulation, FP32
-storage, no contraction), so the north-star tolerance of FP32 relative L2
-<= 1e-5 is met with zero difference.  rel_l2 is reported as well.
+storage, no contraction), so a tolerance of FP32 relative L2 <= 1e-5 is
+met with zero difference.  rel_l2 is reported as well.
 """
 import numpy as np
 import pytest

@@ -404,11 +404,12 @@ def main():
     ap.add_argument("--train", type=int, default=512, help="training scenarios of the surrogate workload")
     ap.add_argument("--calib", type=int, default=32, help="gate-calibration scenarios of the surrogate workload")
     ap.add_argument("--test", type=int, default=64, help="test scenarios of the surrogate workload")
-    ap.add_argument("--epochs", type=int, default=60, help="training epochs of the surrogate workload")
-    ap.add_argument("--fno-width", type=int, default=48, help="FNO channel width of the surrogate workload")
-    ap.add_argument("--fno-modes", type=int, default=12, help="retained Fourier modes per axis")
-    ap.add_argument("--augment", action="store_true", help="also train on the symmetric images of every FD run")
-    ap.add_argument("--sur-batch", type=int, default=16, help="training batch size of the surrogate workload")
+    ap.add_argument("--epochs", type=int, default=70, help="training epochs of the surrogate workload")
+    ap.add_argument("--fno-width", type=int, default=64, help="FNO channel width of the surrogate workload")
+    ap.add_argument("--fno-modes", type=int, default=16, help="retained Fourier modes per axis")
+    ap.add_argument("--augment", action=argparse.BooleanOptionalAction, default=True,
+                    help="also train on the symmetric images of every FD run")
+    ap.add_argument("--sur-batch", type=int, default=32, help="training batch size of the surrogate workload")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--fp64-steps", type=int, default=20, help="steps of the FP64-build error check (0 = skip)")
     ap.add_argument("--zchunks", type=int, default=0)

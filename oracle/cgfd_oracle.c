@@ -40,6 +40,16 @@
 #define M24A1 1.3333333333333333
 #define M24A2 (-0.16666666666666666)
 
+/* Planted mistakes.  The pin tests (tests/test_oracle_pins.py) compile extra
+ * copies of this file with -DORC_PLANT=n, each carrying one plausible
+ * misreading of the reference, to show that some pin fails on it.  The
+ * oracle proper is built with ORC_PLANT = 0, where every block below
+ * compiles to the reference's expression. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(n) (ORC_PLANT == (n))
+
 #define OABS(a) ((a) > 0 ? (a) : -(a))        /* typenew.h:44 */
 #define OMAX(a, b) ((a) >= (b) ? (a) : (b))   /* typenew.h:45 */
 
@@ -78,7 +88,11 @@ static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
 
 /* RK4 coefficients as the reference stores them (calculate.cu:12-13) */
 static const real RK4A[3] = {0.5, 0.5, 1.0};
+#if PLANT(16) /* second and fourth RK weights exchanged */
+static const real RK4B[4] = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0};
+#else
 static const real RK4B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+#endif
 
 /* ---- stencils (typenew.h:58-60, 82-88) ---------------------------------- */
 static inline double drp_fwd(const real *v, long i, long inc, double dh)
@@ -86,24 +100,41 @@ static inline double drp_fwd(const real *v, long i, long inc, double dh)
     return (DA_1 * v[i - 1 * inc] + DA0 * v[i] + DA1 * v[i + 1 * inc] + DA2 * v[i + 2 * inc] +
             DA3 * v[i + 3 * inc]) / dh;
 }
+#if PLANT(15) /* DA2 and DA3 exchanged in the backward sum */
+static inline double drp_bwd(const real *v, long i, long inc, double dh)
+{
+    return (DA2 * v[i + 3 * inc] + DA3 * v[i + 2 * inc] + DA1 * v[i + 1 * inc] + DA0 * v[i] +
+            DA_1 * v[i - 1 * inc]) / dh;
+}
+#else
 static inline double drp_bwd(const real *v, long i, long inc, double dh)
 {
     return (DA3 * v[i + 3 * inc] + DA2 * v[i + 2 * inc] + DA1 * v[i + 1 * inc] + DA0 * v[i] +
             DA_1 * v[i - 1 * inc]) / dh;
 }
+#endif
 static inline real drpfd(const real *v, long i, real dh, long inc)
 {
     return dh > 0 ? (real)drp_fwd(v, i, inc, dh) : (real)drp_bwd(v, i, inc, dh);
 }
 static inline real m24fd(const real *v, long i, real dh, long inc)
 {
+#if PLANT(14) /* backward 2-4 closure divided by |h| */
+    return dh > 0 ? (real)((M24A0 * v[i] + M24A1 * v[i + 1 * inc] + M24A2 * v[i + 2 * inc]) / dh)
+                  : (real)((M24A2 * v[i + 2 * inc] + M24A1 * v[i + 1 * inc] + M24A0 * v[i]) / -dh);
+#else
     return dh > 0 ? (real)((M24A0 * v[i] + M24A1 * v[i + 1 * inc] + M24A2 * v[i + 2 * inc]) / dh)
                   : (real)((M24A2 * v[i + 2 * inc] + M24A1 * v[i + 1 * inc] + M24A0 * v[i]) / dh);
+#endif
 }
 static inline real m22fd(const real *v, long i, real dh, long inc)
 {
     if (dh > 0)
+#if PLANT(13) /* forward two-point difference with its sign flipped */
+        return (v[i] - v[i + 1 * inc]) / dh;
+#else
         return (v[i + 1 * inc] - v[i]) / dh;
+#endif
     return (real)(-1.0 * (v[i] - v[i + 1 * inc]) / dh);
 }
 
@@ -310,7 +341,11 @@ static void velpdcoeff(orc_t *o)
         for (int idy = LFD; idy < o->nj + LFD; idy++) {
             int idz = o->nk + LFD - 1;
             long G = ((long)idx * NY + idy) * NZ + idz;
+#if PLANT(4) /* PML ratio inverted */
+            real Bzx = o->Bt[0][idx] / o->Bt[2][idz];
+#else
             real Bzx = o->Bt[2][idz] / o->Bt[0][idx];
+#endif
             real Bzy = o->Bt[2][idz] / o->Bt[1][idy];
             real miu = o->rho[G] * o->beta[G] * o->beta[G];
             real lam2mu = o->rho[G] * o->alpha[G] * o->alpha[G];
@@ -330,7 +365,11 @@ static void velpdcoeff(orc_t *o)
             A[8] = lam2mu * e33 * e33 + miu * (e31 * e31 + e32 * e32);
             matinv3(A);
             B[0] = lam2mu * e31 * e11 + miu * (e32 * e12 + e33 * e13);
+#if PLANT(3) /* entry (0,1) of the xi block transposed */
+            B[1] = lambda * e32 * e11 + miu * e31 * e12;
+#else
             B[1] = lambda * e31 * e12 + miu * e32 * e11;
+#endif
             B[2] = lambda * e31 * e13 + miu * e33 * e11;
             B[3] = lambda * e32 * e11 + miu * e31 * e12;
             B[4] = lam2mu * e32 * e12 + miu * (e31 * e11 + e33 * e13);
@@ -448,7 +487,11 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
 
                 real DxiVx = (xix * DxTxx + xiy * DxTxy + xiz * DxTxz) * rho;
                 real DetVx = (etx * DyTxx + ety * DyTxy + etz * DyTxz) * rho;
+#if PLANT(1) /* zeta_x and zeta_y swapped in the zeta term of the Vx rate */
+                real DztVx = (zty * DzTxx + ztx * DzTxy + ztz * DzTxz) * rho;
+#else
                 real DztVx = (ztx * DzTxx + zty * DzTxy + ztz * DzTxz) * rho;
+#endif
                 real DxiVy = (xix * DxTxy + xiy * DxTyy + xiz * DxTyz) * rho;
                 real DetVy = (etx * DyTxy + ety * DyTyy + etz * DyTyz) * rho;
                 real DztVy = (ztx * DzTxy + zty * DzTyy + ztz * DzTyz) * rho;
@@ -466,7 +509,11 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
                 real DetTzz = lambda * etx * DyVx + lambda * ety * DyVy + lam2mu * etz * DyVz;
                 real DztTzz = lambda * ztx * DzVx + lambda * zty * DzVy + lam2mu * ztz * DzVz;
                 real DxiTxy = (xiy * DxVx + xix * DxVy) * miu;
+#if PLANT(2) /* eta_x and eta_y swapped in the eta term of the Txy rate */
+                real DetTxy = (etx * DyVx + ety * DyVy) * miu;
+#else
                 real DetTxy = (ety * DyVx + etx * DyVy) * miu;
+#endif
                 real DztTxy = (zty * DzVx + ztx * DzVy) * miu;
                 real DxiTxz = (xiz * DxVx + xix * DxVz) * miu;
                 real DetTxz = (etz * DyVx + etx * DyVz) * miu;
@@ -519,7 +566,13 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
                     hAx[TXZ][P] = DxiTxz * DBx - APDx * Ax[TXZ][P];
                     hAx[TYZ][P] = DxiTyz * DBx - APDx * Ax[TYZ][P];
                     if (surf) {
+#if PLANT(10) /* surface term of the x auxiliary without the Bx factor */
+                        hAx[TXX][P] += DBx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
+#elif PLANT(11) /* the eta part of the closure routed into the x auxiliary */
+                        hAx[TXX][P] += DBx * Bx * (lam2mu * ztx * DzVx2 + lambda * zty * DzVy2 + lambda * ztz * DzVz2);
+#else
                         hAx[TXX][P] += DBx * Bx * (lam2mu * ztx * DzVx1 + lambda * zty * DzVy1 + lambda * ztz * DzVz1);
+#endif
                         hAx[TYY][P] += DBx * Bx * (lambda * ztx * DzVx1 + lam2mu * zty * DzVy1 + lambda * ztz * DzVz1);
                         hAx[TZZ][P] += DBx * Bx * (lambda * ztx * DzVx1 + lambda * zty * DzVy1 + lam2mu * ztz * DzVz1);
                         hAx[TXY][P] += DBx * Bx * (zty * DzVx1 + ztx * DzVy1) * miu;
@@ -540,7 +593,11 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
                     hAy[TXX][P] = DetTxx * DBy - APDy * Ay[TXX][P];
                     hAy[TYY][P] = DetTyy * DBy - APDy * Ay[TYY][P];
                     hAy[TZZ][P] = DetTzz * DBy - APDy * Ay[TZZ][P];
+#if PLANT(12) /* y auxiliary rate of Txy scaled by the x profile */
+                    hAy[TXY][P] = DetTxy * DBx - APDy * Ay[TXY][P];
+#else
                     hAy[TXY][P] = DetTxy * DBy - APDy * Ay[TXY][P];
+#endif
                     hAy[TXZ][P] = DetTxz * DBy - APDy * Ay[TXZ][P];
                     hAy[TYZ][P] = DetTyz * DBy - APDy * Ay[TYZ][P];
                     if (surf) {
@@ -607,10 +664,18 @@ static void timg(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int
                         r = G - LFD * ys + i * ys;
                         VecTy[i] = D[JAC][r] * (D[ETX][r] * T0[r] + D[ETY][r] * T1[r] + D[ETZ][r] * T2[r]);
                         r = G - LFD + i;
+#if PLANT(6) /* Jacobian dropped from the zeta traction vector */
+                        VecTz[i] = (D[ZTX][r] * T0[r] + D[ZTY][r] * T1[r] + D[ZTZ][r] * T2[r]);
+#else
                         VecTz[i] = D[JAC][r] * (D[ZTX][r] * T0[r] + D[ZTY][r] * T1[r] + D[ZTZ][r] * T2[r]);
+#endif
                     }
                     for (int j = 1; j <= LFD - (2 - countZ); j++)
+#if PLANT(5) /* even image above the surface instead of the odd one */
+                        VecTz[LFD + (2 - countZ) + j] = 2.0 * T3Src + VecTz[LFD + (2 - countZ) - j];
+#else
                         VecTz[LFD + (2 - countZ) + j] = 2.0 * T3Src - VecTz[LFD + (2 - countZ) - j];
+#endif
                     VecTz[LFD + (2 - countZ)] = T3Src;
 
                     real DxTx = rhojac * drp_fwd(VecTx, LFD, Xvec, xstep);
@@ -682,7 +747,9 @@ static void sources(orc_t *o, int Tindex)
         real d = 1.0;
         real A = steph * steph * steph * o->drv[JAC][G];
         A = 1 / (A * o->rho[G]) * d;
+#if !PLANT(8) /* PLANT 8: no doubling of a force on the surface row */
         if (idz == o->nk + LFD - 1) A = A * 2.0;
+#endif
         o->hW[VX][G] += stf * o->fvec[3 * mid + 0] * A;
         o->hW[VY][G] += stf * o->fvec[3 * mid + 1] * A;
         o->hW[VZ][G] += stf * o->fvec[3 * mid + 2] * A;
@@ -692,7 +759,11 @@ static void sources(orc_t *o, int Tindex)
         long G = ((long)idx * NY + idy) * NZ + idz;
         real stf = o->mstf[(long)mid * o->nstf_m + Tindex];
         real A = 1.0;
+#if PLANT(7) /* Jacobian dropped from the moment source */
+        A = stf * A / (steph * steph * steph);
+#else
         A = stf * A / (steph * steph * steph * o->drv[JAC][G]);
+#endif
         o->hW[TXX][G] -= o->mvec[6 * mid + 0] * A;
         o->hW[TYY][G] -= o->mvec[6 * mid + 1] * A;
         o->hW[TZZ][G] -= o->mvec[6 * mid + 2] * A;
@@ -797,7 +868,11 @@ static void fb_signs(int currT, int k, int *sx, int *sz)
 {
     int p = currT % 4;
     *sx = (((p >= 2) ? 1 : 0) ^ (k & 1)) ? -1 : 1;
+#if PLANT(9) /* z signs follow the x pattern (wrong in step phases 1 and 3) */
+    *sz = *sx;
+#else
     *sz = (((p == 1 || p == 2) ? 1 : 0) ^ (k & 1)) ? -1 : 1;
+#endif
 }
 
 static void stage(orc_t *o)

@@ -27,7 +27,26 @@ def load(double=False):
     path = os.path.join(HERE, "liborc64.so" if double else "liborc32.so")
     if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(path)
+    _LIBS[key] = _bind(ctypes.CDLL(path), double)
+    return _LIBS[key]
+
+
+def load_planted(plant, outdir, double=True):
+    """Compile and load a copy of the oracle with planted mistake number
+    `plant` (the ORC_PLANT blocks of cgfd_oracle.c) into `outdir`.  Only the
+    pin tests use these copies, to show that a pin fails on a plausible
+    misreading of the reference."""
+    path = os.path.join(outdir, f"liborc{'64' if double else '32'}_plant{int(plant)}.so")
+    if not os.path.exists(path):
+        cmd = ["gcc", "-O2", "-fPIC", "-std=c11", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+               f"-DORC_PLANT={int(plant)}", "-shared", "-o", path, os.path.join(HERE, "cgfd_oracle.c"), "-lm"]
+        if double:
+            cmd.insert(-4, "-DORC_DOUBLE")
+        subprocess.run(cmd, check=True)
+    return _bind(ctypes.CDLL(path), double)
+
+
+def _bind(lib, double):
     R = ctypes.c_double if double else ctypes.c_float
     VP, IP = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
     sig = {
@@ -60,7 +79,6 @@ def load(double=False):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _LIBS[key] = lib
     return lib
 
 
@@ -75,8 +93,8 @@ class OracleError(RuntimeError):
 class Oracle:
     """CPU restatement of the reference time stepping on one whole domain."""
 
-    def __init__(self, ni, nj, nk, dh, dt, nt, double=False):
-        self.lib = load(double)
+    def __init__(self, ni, nj, nk, dh, dt, nt, double=False, lib=None):
+        self.lib = lib if lib is not None else load(double)
         self.dtype = np.float64 if double else np.float32
         self.ni, self.nj, self.nk, self.nt = ni, nj, nk, nt
         self.shape = (ni + 6, nj + 6, nk + 6)

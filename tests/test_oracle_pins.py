@@ -40,8 +40,8 @@ def homogeneous(ni, nj, nk, nt, nabs=(0, 0, 0, 0, 0, 0), pml=None):
     return sc
 
 
-def new_oracle(sc, double=False):
-    o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], double=double)
+def new_oracle(sc, double=False, lib=None):
+    o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], double=double, lib=lib)
     o.setup(sc)
     return o
 
@@ -92,13 +92,17 @@ def test_constant_velocity_gives_coefficient_sum(phase, sign):
 
 @pytest.mark.parametrize("phase,forward", [(0, True), (0, False)])
 def test_linear_stress_gives_first_moment(phase, forward):
+    pin_linear_stress_first_moment(forward)
+
+
+def pin_linear_stress_first_moment(forward, lib=None):
     """Txx = s*i (i the x index): the forward x-derivative is
     s*(i*sum(A) + M1)/h and the backward one s*(M1 - i*sum(A))/h with
     M1 = sum(k*A_k); hVx = dTxx/dx / rho.  Stage 0 of step 0 is forward in
     x, stage 1 backward."""
     ni, nj, nk = 30, 20, 20
     sc = homogeneous(ni, nj, nk, 4)
-    o = new_oracle(sc, double=True)
+    o = new_oracle(sc, double=True, lib=lib)
     s = 1e4
     t = np.zeros(o.shape)
     i = np.arange(o.shape[0], dtype=np.float64)[:, None, None] - 3
@@ -131,6 +135,10 @@ def test_free_surface_closure_flat():
 
 
 def test_rk4_amplification_on_auxiliary_ode():
+    pin_rk4_amplification()
+
+
+def pin_rk4_amplification(lib=None):
     """With d = 0 (DB = 0) and a constant APD = g, each auxiliary field obeys
     dA/dt = -g A independently of the wavefield, so one step multiplies it by
     the classical RK4 polynomial 1 + z + z^2/2 + z^3/6 + z^4/24, z = -g dt."""
@@ -141,7 +149,7 @@ def test_rk4_amplification_on_auxiliary_ode():
     for nm in "xyz":
         pml["APD" + nm][:] = g
     sc = homogeneous(ni, nj, nk, 2, nabs, pml)
-    o = new_oracle(sc, double=True)
+    o = new_oracle(sc, double=True, lib=lib)
     rng = np.random.default_rng(0)
     start = {}
     for axis in range(3):

@@ -174,14 +174,16 @@ static void fb_signs(int t, int k, int *sx, int *sz)
 }
 
 // z chunks per column sweep: enough workgroups to fill the chip (~2048 of
-// 64x4 tiles), but at least 2 planes per chunk.  Small grids want many short
-// chunks (64^3: 2.85 -> 4.81 Gcell-RKstages/s going from 8 to 2 planes); the
-// 600x600x300 model gets 2 chunks of 150 planes.
+// 64x4 tiles) and at most ~32 planes per chunk, but at least 2 planes per
+// chunk.  Small grids want many short chunks (64^3: 2.88 -> 4.73
+// Gcell-RKstages/s going from 8 to 2 planes); short chunks also keep the
+// launch's tail short (600x600x300: 6.07 ms per stage with 2 chunks of 150
+// planes, 5.59 ms with 10 of 30; 200x200x150 is fastest at 11 of 14).
 static void default_tuning(cgfd_ctx *c)
 {
     c->rows = 4;
     const long tiles = (long)((c->g.ni + 63) / 64) * ((c->g.nj + c->rows - 1) / c->rows);
-    long zc = (2048 + tiles - 1) / tiles;
+    long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 31) / 32);
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
 }
@@ -1113,10 +1115,10 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
     return most < 0x7fffffffL;
 }
 
-// Workgroups of a batched stage launch: every grid sweeps its columns in the
-// same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS workgroups
-// (tiles of 64 columns x stage_tile_rows rows) over all grids and clamped to
-// 2 planes per chunk per grid.
+// Workgroups of a batched stage launch: every grid sweeps its columns in
+// about the same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS
+// workgroups (tiles of 64 columns x stage_tile_rows rows) over all grids,
+// with at most ~32 and at least 2 planes per chunk per grid.
 // zlen[c] is grid c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
@@ -1130,7 +1132,7 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, std::vector<int> &zlen, s
     starts.assign(n + 1, 0);
     for (int i = 0; i < n; i++) {
         const int nk = ctxs[i]->g.nk;
-        const int zci = (int)std::min(zc, (long)std::max(1, nk / 2));
+        const int zci = (int)std::min(std::max(zc, (long)(nk + 31) / 32), (long)std::max(1, nk / 2));
         zlen[i] = (nk + zci - 1) / zci;
         starts[i + 1] = starts[i] + (int)(batch_tiles(ctxs[i]) * zci);
     }
@@ -1638,7 +1640,12 @@ int cgfd_set_variant(cgfd_ctx *c, int variant)
     if (!c) return CGFD_EINVAL;
     if (variant < 0 || variant > 2)
         return fail(c, CGFD_EINVAL, "variant must be 0 (LDS), 1 (direct) or 2 (split velocity/stress)");
-    c->variant = variant;
+    if (variant != c->variant) {
+        c->variant = variant;
+        // captured cycles launch the old variant's kernels: the next run
+        // rewrites the argument image, which drops them after a stream sync
+        c->args_dirty = true;
+    }
     return 0;
 }
 

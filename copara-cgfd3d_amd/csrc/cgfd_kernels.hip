@@ -225,16 +225,15 @@ __device__ __forceinline__ void store_aux(AP a, int ax, long aidx, const AuxIn<R
     }
 }
 
-// Everything of one cell after its 27 derivatives are known: the free-surface
-// velocity closure, CalWaveCL, the ADE CFS-PML rates with their RK update,
-// the traction-image override, the point sources and the RK update of the
-// wavefield (plus the PGV maximum in the last stage).
+// Everything of one cell after its 27 derivatives are known, up to its nine
+// rates hv: the free-surface velocity closure, CalWaveCL, the ADE CFS-PML
+// rates with their RK update, the traction-image override and the point
+// sources.  The RK update of the wavefield is rk_store (below).
 //   w: z window of the stage input (slot CI is the cell itself)
-//   pb: byte offset of the cell inside a field
 template <typename R, int SX, int SZ, int ST, typename AP>
-__device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const CellIn<R> &in, R (&Dx)[NF], R (&Dy)[NF],
-                                            R (&Dz)[NF], const R (&w)[NF][5], int x, int y, int z, unsigned pb,
-                                            int &e, long fs, int tindex)
+__device__ __forceinline__ void cell_rates(AP a, const Column<R> &k, const CellIn<R> &in, R (&Dx)[NF], R (&Dy)[NF],
+                                           R (&Dz)[NF], const R (&w)[NF][5], int x, int y, int z, int &e,
+                                           int tindex, R (&hv)[NF])
 {
     const Geom g = geom_of(a);
     const double h = a->h;
@@ -308,29 +307,27 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
     const R *pmz = a->pml + 3 * g.ni + 3 * g.nj;
     const R APDz = pmz[z], Bz = pmz[g.nk + z], DBz = pmz[2 * g.nk + z];
     const R Bx = k.Bx, By = k.By;
-    R hv[NF];
-    if (Bx == (R)1 && By == (R)1 && Bz == (R)1) {
-        // x/1 == x exactly: the interior skips the nine divisions
-        hv[TXX] = DxiTxx + DetTxx + DztTxx;
-        hv[TYY] = DxiTyy + DetTyy + DztTyy;
-        hv[TZZ] = DxiTzz + DetTzz + DztTzz;
-        hv[TXY] = DxiTxy + DetTxy + DztTxy;
-        hv[TXZ] = DxiTxz + DetTxz + DztTxz;
-        hv[TYZ] = DxiTyz + DetTyz + DztTyz;
-        hv[VX] = DxiVx + DetVx + DztVx;
-        hv[VY] = DxiVy + DetVy + DztVy;
-        hv[VZ] = DxiVz + DetVz + DztVz;
-    } else {
-        hv[TXX] = DxiTxx / Bx + DetTxx / By + DztTxx / Bz;
-        hv[TYY] = DxiTyy / Bx + DetTyy / By + DztTyy / Bz;
-        hv[TZZ] = DxiTzz / Bx + DetTzz / By + DztTzz / Bz;
-        hv[TXY] = DxiTxy / Bx + DetTxy / By + DztTxy / Bz;
-        hv[TXZ] = DxiTxz / Bx + DetTxz / By + DztTxz / Bz;
-        hv[TYZ] = DxiTyz / Bx + DetTyz / By + DztTyz / Bz;
-        hv[VX] = DxiVx / Bx + DetVx / By + DztVx / Bz;
-        hv[VY] = DxiVy / Bx + DetVy / By + DztVy / Bz;
-        hv[VZ] = DxiVz / Bx + DetVz / By + DztVz / Bz;
+    // CalWaveCL divides each axis' term by that axis' CFS factor B.  x/1 is
+    // x exactly, so an axis whose B is 1 for every lane of the wavefront
+    // (all of the interior; By is uniform over a wavefront row and Bz over a
+    // plane) skips its nine divisions; the sums keep the reference's order.
+    R qx[NF] = {DxiVx, DxiVy, DxiVz, DxiTxx, DxiTyy, DxiTzz, DxiTxy, DxiTxz, DxiTyz};
+    R qy[NF] = {DetVx, DetVy, DetVz, DetTxx, DetTyy, DetTzz, DetTxy, DetTxz, DetTyz};
+    R qz[NF] = {DztVx, DztVy, DztVz, DztTxx, DztTyy, DztTzz, DztTxy, DztTxz, DztTyz};
+    if (Bx != (R)1) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qx[c] = qx[c] / Bx;
     }
+    if (By != (R)1) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qy[c] = qy[c] / By;
+    }
+    if (Bz != (R)1) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qz[c] = qz[c] / Bz;
+    }
+#pragma unroll
+    for (int c = 0; c < NF; ++c) hv[c] = qx[c] + qy[c] + qz[c];
 
     const unsigned tcell = (unsigned)(top > 0 ? top : 0) * (unsigned)(g.nj * g.nxp) + k.cxy; // top-row slot
 
@@ -467,25 +464,55 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
         }
     }
 
-    // ---- RK4 update (Iteration*, calculate.cu:6330-6852) -------------------
+}
+
+// RK4 update of the wavefield (Iteration*, calculate.cu:6330-6852) from the
+// cell's rates hv.  The stage-0 input is FW itself (w[c][CI]).
+template <typename R, int SZ, int ST, typename AP>
+__device__ __forceinline__ void rk_store(AP a, const CellIn<R> &in, const R (&w)[NF][5], const R (&hv)[NF],
+                                         unsigned pb, long fs)
+{
+    constexpr int CI = SZ > 0 ? 1 : 3;
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         if (ST == 0) {
-            const R f0 = w[c][CI]; // the stage input is FW itself
+            const R f0 = w[c][CI];
             stg_once(a->wout + c * fs, pb, f0 + a->alpha * hv[c]);
             stg_once(a->tw + c * fs, pb, f0 + a->beta * hv[c]);
         } else if (ST < 3) {
             stg_once(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c]);
             stg_once(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c]);
         } else {
-            const R wn = in.tw[c] + a->beta * hv[c];
-            stg_once(a->wout + c * fs, pb, wn);
-            if (c < 3 && top == 2 && a->pgv) {
-                R *pv = a->pgv + c * a->cstr + k.cxy; // PGV maps share the [nj][nxp] slot size of cov
-                *pv = rmax(rabs(wn), rabs(*pv));
-            }
+            stg_once(a->wout + c * fs, pb, in.tw[c] + a->beta * hv[c]);
         }
     }
+}
+
+// peak ground velocity of the last stage: running max |V| of the surface
+// row (IterationFinalPV, calculate.cu:6689-6694), from the stored values
+template <typename R, int ST, typename AP>
+__device__ __forceinline__ void pgv_update(AP a, const Column<R> &k, const CellIn<R> &in, const R (&hv)[NF], int top)
+{
+    if (ST == 3 && top == 2 && a->pgv) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const R wn = in.tw[c] + a->beta * hv[c];
+            R *pv = a->pgv + c * a->cstr + k.cxy; // PGV maps share the [nj][nxp] slot size of cov
+            *pv = rmax(rabs(wn), rabs(*pv));
+        }
+    }
+}
+
+// the whole cell: rates, RK update, PGV (lanes of valid cells only)
+template <typename R, int SX, int SZ, int ST, typename AP>
+__device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const CellIn<R> &in, R (&Dx)[NF], R (&Dy)[NF],
+                                            R (&Dz)[NF], const R (&w)[NF][5], int x, int y, int z, unsigned pb,
+                                            int &e, long fs, int tindex)
+{
+    R hv[NF];
+    cell_rates<R, SX, SZ, ST>(a, k, in, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
+    rk_store<R, SZ, ST>(a, in, w, hv, pb, fs);
+    pgv_update<R, ST>(a, k, in, hv, z - (geom_of(a).nk - 3));
 }
 
 // Variant "direct": the x and y stencil taps are read straight from memory
@@ -740,9 +767,15 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         asm volatile("" : "+s"(fs));
         AP al = a0;
         asm volatile("" : "+s"(al));
+        // Every lane loads its cell inputs and stores its results without a
+        // branch: a lane outside the grid addresses its own zero guard cell
+        // (x in [ni, nxp) or a clamped guard row), reads zeros and stores
+        // zeros back.  With no skippable memory operation between them, the
+        // compiler's wait counts let this plane's cell loads stay in flight
+        // while the previous plane's prefetch is consumed.
         const unsigned pbz = pbase(z) + rel_c;
         CellIn<R> cin;
-        if (active) load_cell<R, ST, HM>(al, pbz, fs, cin);
+        load_cell<R, ST, HM>(al, pbz, fs, cin);
         __syncthreads(); // the previous plane's LDS reads are done (its tile is slot(z+HD) now)
         const int sc = slot(z), sh = slot(z + HD);
 #pragma unroll
@@ -761,6 +794,9 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
             h1[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_1);
             h2[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_2);
         }
+        R hv[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) hv[c] = (R)0;
         if (active) {
             R Dx[NF], Dy[NF], Dz[NF];
 #pragma unroll
@@ -777,8 +813,11 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
                 }
                 Dz[c] = dfd<R, SZ>(w[c][0], w[c][1], w[c][2], w[c][3], w[c][4], h, rh);
             }
-            finish_cell<R, SX, SZ, ST>(al, k, cin, Dx, Dy, Dz, w, x, y, z, pbz, e, fs, tindex);
+            cell_rates<R, SX, SZ, ST>(al, k, cin, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
         }
+        // guard lanes: zero inputs and zero rates store zeros
+        rk_store<R, SZ, ST>(al, cin, w, hv, pbz, fs);
+        if (active) pgv_update<R, ST>(al, k, cin, hv, z - (g.nk - 3));
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             w[c][0] = w[c][1];

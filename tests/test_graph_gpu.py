@@ -77,6 +77,27 @@ def test_graph_recaptures_after_tuning_change():
     assert_same(g.get_seismo(), e.get_seismo(), "seismo")
 
 
+def test_graph_recaptures_after_variant_change():
+    # a variant switch in mid-run drops the captured cycles, so the replays
+    # after it launch the new variant's kernels (all variants give the same
+    # values, so the state must still equal the eager run's)
+    sc = _scenario(24)
+    e = _sim(sc)
+    e.run(24)
+    g = _sim(sc)
+    g.set_graph(True)
+    g.run(8)
+    g.set_variant(2)
+    g.run(8)
+    g.set_variant(1)
+    g.run(4)
+    g.set_variant(0)
+    g.run(4)
+    for c in range(9):
+        assert_same(g.get_field(c), e.get_field(c), f"field {c}")
+    assert_same(g.get_seismo(), e.get_seismo(), "seismo")
+
+
 def test_batch_with_graphs_equals_single_eager_runs():
     # contexts of different sizes and metric modes (so one stream each)
     # advanced by cgfd_run_batch with graph replay: 2 whole cycles, then 2

@@ -55,6 +55,7 @@ SIGNATURES = {
     "cgfd_sync": (ctypes.c_int, [_VP]),
     "cgfd_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_batch": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
+    "cgfd_run_batch_static": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
     "cgfd_batch_fusable": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int]),
     "cgfd_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP]),
     "cgfd_current_step": (ctypes.c_int, [_VP]),
@@ -163,6 +164,19 @@ def run_batch(sims, nsteps):
     if rc:
         msgs = "; ".join(lib.cgfd_last_error(s.h).decode() for s in sims)
         raise CgfdError(f"cgfd_run_batch failed ({rc}): {msgs}")
+
+
+def run_batch_static(sims, nsteps):
+    """run_batch's fused path without any device allocation (the library's
+    fixed per-device array holds the batch's argument images): at most 64
+    Simulations that batch_fusable accepts."""
+    if not sims:
+        return
+    lib = sims[0].lib
+    arr = (_VP * len(sims))(*[s.h for s in sims])
+    rc = lib.cgfd_run_batch_static(arr, len(sims), int(nsteps))
+    if rc:
+        raise CgfdError(f"cgfd_run_batch_static failed ({rc}): {lib.cgfd_last_error(sims[0].h).decode()}")
 
 
 def batch_fusable(sims):

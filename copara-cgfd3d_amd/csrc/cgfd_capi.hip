@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -254,10 +255,15 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
         (rc = dalloc(c, &c->timg, (size_t)9 * c->tstr * c->esz)) ||
         (rc = dalloc(c, &c->cov, (size_t)18 * c->cstr * c->esz)) ||
         (rc = dalloc(c, &c->pgv, (size_t)3 * c->cstr * c->esz)) ||
-        (rc = dalloc(c, (char **)&c->d_step, sizeof(int)))) {
+        (rc = dalloc(c, (char **)&c->d_step, sizeof(int))) || (rc = dalloc(c, &c->args_d, kCtxArgBytes))) {
         fprintf(stderr, "cgfd_create: %s\n", c->err.c_str());
         cgfd_destroy(c);
         return rc;
+    }
+    if (hipEventCreateWithFlags(&c->bev, hipEventDisableTiming) != hipSuccess) {
+        fprintf(stderr, "cgfd_create: hipEventCreateWithFlags failed\n");
+        cgfd_destroy(c);
+        return CGFD_EHIP;
     }
     // unit scaling factors until cgfd_set_pml supplies tables
     if (precision == CGFD_FP32) {
@@ -1123,13 +1129,12 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
 #endif
-static void batch_layout(cgfd_ctx *const *ctxs, int n, std::vector<int> &zlen, std::vector<int> &starts)
+static void batch_layout(cgfd_ctx *const *ctxs, int n, int *zlen, int *starts)
 {
     long tiles = 0;
     for (int i = 0; i < n; i++) tiles += batch_tiles(ctxs[i]);
     const long zc = std::max(1L, (CGFD_BATCH_WORKGROUPS + tiles - 1) / tiles);
-    zlen.assign(n, 0);
-    starts.assign(n + 1, 0);
+    starts[0] = 0;
     for (int i = 0; i < n; i++) {
         const int nk = ctxs[i]->g.nk;
         const int zci = (int)std::min(std::max(zc, (long)(nk + 31) / 32), (long)std::max(1, nk / 2));
@@ -1138,53 +1143,113 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, std::vector<int> &zlen, s
     }
 }
 
-template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, int nsteps)
+// The device side of the allocation-free batch entry point: one fixed
+// array per device for the argument images of up to kStaticBatchMax grids
+// and their workgroup starts.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+constexpr int kStaticBatchMax = 64;
+constexpr size_t kStaticBatchBytes =
+    ((size_t)kStaticBatchMax * kCtxArgBytes + (kStaticBatchMax + 1) * sizeof(int) + 255) / 256 * 256;
+static __device__ __align__(256) char scratch[kStaticBatchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct ScratchDev {
+    bool ready = false;
+    char *ptr = nullptr;       // device address of `scratch` on this device
+    hipEvent_t done = nullptr; // recorded after the last launch that read it
+    bool pending = false;
+};
+ScratchDev g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+} // namespace
+
+// per-device set-up on the first call: the array's device address and the
+// event that orders rewrites of it after the previous call's launches
+static int scratch_setup(cgfd_ctx *c, ScratchDev **out)
+{
+    if (c->dev < 0 || c->dev >= kMaxDevices) return fail(c, CGFD_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchDev &sd = g_scratch[c->dev];
+    if (!sd.ready) {
+        void *ptr = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        HIPCHK(c, hipEventCreateWithFlags(&sd.done, hipEventDisableTiming));
+        sd.ptr = (char *)ptr;
+        sd.ready = true;
+    }
+    *out = &sd;
+    return 0;
+}
+
+template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, int nsteps, ScratchDev *fixed)
 {
     cgfd_ctx *c0 = ctxs[0];
     HIPCHK(c0, hipSetDevice(c0->dev));
-    for (int i = 0; i < n; i++)
-        if (ctxs[i]->args_dirty) {
-            int rc = upload_args(ctxs[i]);
-            if (rc) return rc;
-        }
-    // device image: the grids' argument images, then the workgroup starts
-    std::vector<int> zlen, starts;
-    batch_layout(ctxs, n, zlen, starts);
+    int zlen[kStaticBatchMax], starts_fixed[kStaticBatchMax + 1];
+    std::vector<int> zv, sv;
+    int *zl = zlen, *starts = starts_fixed;
+    if (!fixed) {
+        for (int i = 0; i < n; i++)
+            if (ctxs[i]->args_dirty) {
+                int rc = upload_args(ctxs[i]);
+                if (rc) return rc;
+            }
+        zv.resize(n);
+        sv.resize(n + 1);
+        zl = zv.data();
+        starts = sv.data();
+    }
+    batch_layout(ctxs, n, zl, starts);
     const size_t sofs = (size_t)n * kCtxArgBytes;
-    std::vector<char> img(sofs + starts.size() * sizeof(int));
     int max_nrecv = 0, ni = 0, nj = 0;
     long cols = 0;
     for (int i = 0; i < n; i++) {
         const cgfd_ctx *c = ctxs[i];
-        make_image<R>(c, img.data() + (size_t)i * kCtxArgBytes, zlen[i]);
         max_nrecv = std::max(max_nrecv, c->nrecv);
         ni = std::max(ni, c->g.ni);
         nj = std::max(nj, c->g.nj);
         cols += (long)c->g.ni * c->g.nj;
     }
-    memcpy(img.data() + sofs, starts.data(), starts.size() * sizeof(int));
-    if (img != c0->bimg_h) {
-        // launches of an earlier batch on c0's stream may still read the image
-        HIPCHK(c0, hipStreamSynchronize(c0->stream));
-        if (c0->bimg_cap < img.size()) {
-            dfree(c0->bimg_d);
-            c0->bimg_cap = 0;
-            int rc = dalloc(c0, &c0->bimg_d, img.size());
-            if (rc) return rc;
-            c0->bimg_cap = img.size();
+    const char *img_d = nullptr;
+    if (fixed) {
+        // the previous call's launches may still read the array; then each
+        // grid's image goes over from a stack buffer (nothing is allocated)
+        if (fixed->pending) HIPCHK(c0, hipEventSynchronize(fixed->done));
+        fixed->pending = false;
+        alignas(16) char one[kCtxArgBytes];
+        for (int i = 0; i < n; i++) {
+            make_image<R>(ctxs[i], one, zl[i]);
+            HIPCHK(c0, hipMemcpy(fixed->ptr + (size_t)i * kCtxArgBytes, one, kCtxArgBytes, hipMemcpyHostToDevice));
         }
-        HIPCHK(c0, hipMemcpy(c0->bimg_d, img.data(), img.size(), hipMemcpyHostToDevice));
-        c0->bimg_h.swap(img);
+        HIPCHK(c0, hipMemcpy(fixed->ptr + sofs, starts, (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        img_d = fixed->ptr;
+    } else {
+        // device image: the grids' argument images, then the workgroup starts
+        std::vector<char> img(sofs + (n + 1) * sizeof(int));
+        for (int i = 0; i < n; i++) make_image<R>(ctxs[i], img.data() + (size_t)i * kCtxArgBytes, zl[i]);
+        memcpy(img.data() + sofs, starts, (n + 1) * sizeof(int));
+        if (img != c0->bimg_h) {
+            // launches of an earlier batch on c0's stream may still read the image
+            HIPCHK(c0, hipStreamSynchronize(c0->stream));
+            if (c0->bimg_cap < img.size()) {
+                dfree(c0->bimg_d);
+                c0->bimg_cap = 0;
+                int rc = dalloc(c0, &c0->bimg_d, img.size());
+                if (rc) return rc;
+                c0->bimg_cap = img.size();
+            }
+            HIPCHK(c0, hipMemcpy(c0->bimg_d, img.data(), img.size(), hipMemcpyHostToDevice));
+            c0->bimg_h.swap(img);
+        }
+        img_d = c0->bimg_d;
     }
     // c0's stream runs the batch: it waits for work queued on the others
-    for (int i = 0; i < n; i++) {
+    for (int i = 1; i < n; i++) {
         cgfd_ctx *c = ctxs[i];
-        if (!c->bev) HIPCHK(c, hipEventCreateWithFlags(&c->bev, hipEventDisableTiming));
-        if (i == 0) continue;
         HIPCHK(c, hipEventRecord(c->bev, c->stream));
         HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->bev, 0));
     }
-    const char *img_d = c0->bimg_d;
     const int *starts_d = (const int *)(img_d + sofs);
     const int hmet = c0->hmet && c0->use_hmet ? 1 : 0;
     for (int t = 0; t < nsteps; t++) {
@@ -1214,6 +1279,10 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
     // the other streams wait for the batch before their own later work
     HIPCHK(c0, hipEventRecord(c0->bev, c0->stream));
     for (int i = 1; i < n; i++) HIPCHK(ctxs[i], hipStreamWaitEvent(ctxs[i]->stream, c0->bev, 0));
+    if (fixed) {
+        HIPCHK(c0, hipEventRecord(fixed->done, c0->stream));
+        fixed->pending = true;
+    }
     return 0;
 }
 
@@ -1239,8 +1308,8 @@ int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
             return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
     }
     if (nsteps > 0 && batch_fusable(ctxs, n))
-        return ctxs[0]->prec == CGFD_FP32 ? run_batch_fused<float>(ctxs, n, nsteps)
-                                          : run_batch_fused<double>(ctxs, n, nsteps);
+        return ctxs[0]->prec == CGFD_FP32 ? run_batch_fused<float>(ctxs, n, nsteps, nullptr)
+                                          : run_batch_fused<double>(ctxs, n, nsteps, nullptr);
     bool graphs = n > 0;
     for (int i = 0; i < n; i++) graphs = graphs && ctxs[i]->use_graph && !ctxs[i]->prof;
     if (graphs) {
@@ -1263,6 +1332,41 @@ int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps)
                 if (rc) return rc;
             }
     return 0;
+}
+
+// The fused batch of cgfd_run_batch without allocating: the batch's argument
+// images live in the library's fixed per-device array (kStaticBatchMax
+// grids), the contexts' events exist since cgfd_create, and the images are
+// written from the stack.  Calls that overlap on one device share the array
+// and race; a call waits for the previous call's launches before it
+// rewrites the array.
+int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps)
+{
+    if (!ctxs || n < 0 || nsteps < 0) return CGFD_EINVAL;
+    for (int i = 0; i < n; i++) {
+        cgfd_ctx *c = ctxs[i];
+        if (!c) return CGFD_EINVAL;
+        if (!c->inited) return fail(c, CGFD_ESTATE, "cgfd_init must run after the last set_* call");
+        if (c->stage != 0) return fail(c, CGFD_ESTATE, "cgfd_run_batch_static called in the middle of a step");
+        if ((long)c->currT + nsteps > c->d.nt)
+            return fail(c, CGFD_ESTATE, "cannot step past nt (the source tables and traces are sized for nt steps)");
+    }
+    if (n == 0 || nsteps == 0) return 0;
+    if (n > kStaticBatchMax)
+        return fail(ctxs[0], CGFD_EINVAL,
+                    "cgfd_run_batch_static takes at most " + std::to_string(kStaticBatchMax) + " grids, got " +
+                        std::to_string(n));
+    if (!batch_fusable(ctxs, n))
+        return fail(ctxs[0], CGFD_EINVAL,
+                    "cgfd_run_batch_static needs a list that cgfd_batch_fusable accepts (2 or more grids on one "
+                    "device, same precision, default variant, metric mode and step phase, none profiling)");
+    HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->dev));
+    ScratchDev *sd = nullptr;
+    int rc = scratch_setup(ctxs[0], &sd);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return ctxs[0]->prec == CGFD_FP32 ? run_batch_fused<float>(ctxs, n, nsteps, sd)
+                                      : run_batch_fused<double>(ctxs, n, nsteps, sd);
 }
 
 // Autotuner: time `trial_steps` steps for each candidate z-chunk count

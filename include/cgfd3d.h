@@ -128,6 +128,19 @@ int cgfd_run_stages(cgfd_ctx *ctx, int nstages);
  * enqueued on its own stream before stage k+1 of any, so small grids share
  * the GPU.  Results equal those of running each context alone. */
 int cgfd_run_batch(cgfd_ctx *const *ctxs, int n, int nsteps);
+/* No counterpart in the reference: cgfd_run_batch's fused path for callers
+ * that cannot allocate device memory (code under graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The list must be one that
+ * cgfd_batch_fusable accepts, of at most 64 grids; the batch's argument
+ * images go into a fixed array of the library (513 KiB per device, held
+ * until the process ends) instead of a buffer it allocates, and nothing is
+ * allocated.  Results equal cgfd_run_batch's.  Every call on a device shares
+ * that device's array: a call waits for the previous call's launches before
+ * it rewrites it, but calls that overlap on one device (from several host
+ * threads, or while another call is being set up) race.  CGFD_EINVAL for a
+ * longer or unfusable list, with the reason in cgfd_last_error of the first
+ * context. */
+int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
 /* 1 if cgfd_run_batch runs this list as one launch per kernel (no reference
  * counterpart): 2 to 4096 distinct contexts on one device with the same
  * precision, the default stage variant, the same metric mode and the same

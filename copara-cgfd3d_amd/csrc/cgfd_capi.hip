@@ -175,16 +175,17 @@ static void fb_signs(int t, int k, int *sx, int *sz)
 }
 
 // z chunks per column sweep: enough workgroups to fill the chip (~2048 of
-// 64x4 tiles) and at most ~32 planes per chunk, but at least 2 planes per
+// 64x4 tiles) and at most ~25 planes per chunk, but at least 2 planes per
 // chunk.  Small grids want many short chunks (64^3: 2.88 -> 4.73
 // Gcell-RKstages/s going from 8 to 2 planes); short chunks also keep the
 // launch's tail short (600x600x300: 6.07 ms per stage with 2 chunks of 150
-// planes, 5.59 ms with 10 of 30; 200x200x150 is fastest at 11 of 14).
+// planes, 5.59 ms with 10 of 30, 5.30 ms with 12 of 25 after later changes;
+// 200x200x150 is fastest at 11 chunks of 14 planes).
 static void default_tuning(cgfd_ctx *c)
 {
     c->rows = 4;
     const long tiles = (long)((c->g.ni + 63) / 64) * ((c->g.nj + c->rows - 1) / c->rows);
-    long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 31) / 32);
+    long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 24) / 25);
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
 }
@@ -1124,7 +1125,7 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 // Workgroups of a batched stage launch: every grid sweeps its columns in
 // about the same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS
 // workgroups (tiles of 64 columns x stage_tile_rows rows) over all grids,
-// with at most ~32 and at least 2 planes per chunk per grid.
+// with at most ~25 and at least 2 planes per chunk per grid.
 // zlen[c] is grid c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
@@ -1137,7 +1138,7 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, int *zlen, int *starts)
     starts[0] = 0;
     for (int i = 0; i < n; i++) {
         const int nk = ctxs[i]->g.nk;
-        const int zci = (int)std::min(std::max(zc, (long)(nk + 31) / 32), (long)std::max(1, nk / 2));
+        const int zci = (int)std::min(std::max(zc, (long)(nk + 24) / 25), (long)std::max(1, nk / 2));
         zlen[i] = (nk + zci - 1) / zci;
         starts[i + 1] = starts[i] + (int)(batch_tiles(ctxs[i]) * zci);
     }

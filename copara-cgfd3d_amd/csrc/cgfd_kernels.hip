@@ -50,9 +50,26 @@ template <typename R> __device__ __forceinline__ void stg(R *base, unsigned off,
 #ifndef CGFD_NT
 #define CGFD_NT 1
 #endif
+// FP32 single-use streams go through buffer instructions: the field base
+// becomes a resource descriptor formed on the scalar unit, and every load or
+// store of a cell shares the one 32-bit lane offset, so no per-lane 64-bit
+// address is formed.  The descriptor's record count is the full 32-bit
+// range, which no offset here reaches (offsets stay inside one field).
+#ifndef CGFD_BUFFER_OPS
+#define CGFD_BUFFER_OPS 0
+#endif
+constexpr int kBufNT = 2; // cache policy of the buffer instructions: nontemporal
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, -1, 0x00020000);
+}
 template <typename R> __device__ __forceinline__ R ldg_once(const R *base, unsigned off)
 {
 #if CGFD_NT
+#if CGFD_BUFFER_OPS
+    if constexpr (sizeof(R) == 4)
+        return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base), off, 0, kBufNT));
+#endif
     return __builtin_nontemporal_load((const R *)((const char *)base + off));
 #else
     return ldg(base, off);
@@ -61,6 +78,12 @@ template <typename R> __device__ __forceinline__ R ldg_once(const R *base, unsig
 template <typename R> __device__ __forceinline__ void stg_once(R *base, unsigned off, R v)
 {
 #if CGFD_NT
+#if CGFD_BUFFER_OPS
+    if constexpr (sizeof(R) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), buf_rsrc(base), off, 0, kBufNT);
+        return;
+    }
+#endif
     __builtin_nontemporal_store(v, (R *)((char *)base + off));
 #else
     stg(base, off, v);
@@ -196,11 +219,12 @@ template <typename R, int ST, typename AP>
 __device__ __forceinline__ void load_aux(AP a, int ax, long aidx, AuxIn<R> &v)
 {
     const long as = a->astr[ax];
+    const unsigned off = (unsigned)(aidx * (long)sizeof(R)); // one auxiliary component is far below 4 GiB
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
-        v.ai[c] = ld_once(a->ain[ax] + c * as + aidx);
-        if (ST == 1 || ST == 2) v.af[c] = ld_once(a->af[ax] + c * as + aidx);
-        if (ST >= 1) v.at[c] = ld_once((const R *)a->at[ax] + c * as + aidx);
+        v.ai[c] = ldg_once(a->ain[ax] + c * as, off);
+        if (ST == 1 || ST == 2) v.af[c] = ldg_once(a->af[ax] + c * as, off);
+        if (ST >= 1) v.at[c] = ldg_once((const R *)a->at[ax] + c * as, off);
     }
 }
 
@@ -209,18 +233,19 @@ template <typename R, int ST, typename AP>
 __device__ __forceinline__ void store_aux(AP a, int ax, long aidx, const AuxIn<R> &v, const R (&ha)[NF])
 {
     const long as = a->astr[ax];
-    R *ao = a->aout[ax] + aidx;
-    R *at = a->at[ax] + aidx;
+    const unsigned off = (unsigned)(aidx * (long)sizeof(R));
+    R *ao = a->aout[ax];
+    R *at = a->at[ax];
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         if (ST == 0) {
-            st_once(ao + c * as, v.ai[c] + a->alpha * ha[c]);
-            st_once(at + c * as, v.ai[c] + a->beta * ha[c]);
+            stg_once(ao + c * as, off, v.ai[c] + a->alpha * ha[c]);
+            stg_once(at + c * as, off, v.ai[c] + a->beta * ha[c]);
         } else if (ST < 3) {
-            st_once(ao + c * as, v.af[c] + a->alpha * ha[c]);
-            st_once(at + c * as, v.at[c] + a->beta * ha[c]);
+            stg_once(ao + c * as, off, v.af[c] + a->alpha * ha[c]);
+            stg_once(at + c * as, off, v.at[c] + a->beta * ha[c]);
         } else {
-            st_once(ao + c * as, v.at[c] + a->beta * ha[c]);
+            stg_once(ao + c * as, off, v.at[c] + a->beta * ha[c]);
         }
     }
 }
@@ -466,24 +491,42 @@ __device__ __forceinline__ void cell_rates(AP a, const Column<R> &k, const CellI
 
 }
 
+// store of the RK update: in FP32 a buffer store whose lane offset lies
+// past the field (keep = false) is dropped by the hardware, so lanes outside
+// the grid store nothing and no branch is needed; FP64 lanes outside the
+// grid store the zeros they computed into their own guard cell
+template <typename R>
+__device__ __forceinline__ void stg_keep(R *base, unsigned off, R v, bool keep, unsigned bytes)
+{
+#if CGFD_NT
+    if constexpr (sizeof(R) == 4) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, keep ? off : 0xfffffff0u, 0, kBufNT);
+        return;
+    }
+#endif
+    stg_once(base, off, v);
+}
+
 // RK4 update of the wavefield (Iteration*, calculate.cu:6330-6852) from the
 // cell's rates hv.  The stage-0 input is FW itself (w[c][CI]).
 template <typename R, int SZ, int ST, typename AP>
 __device__ __forceinline__ void rk_store(AP a, const CellIn<R> &in, const R (&w)[NF][5], const R (&hv)[NF],
-                                         unsigned pb, long fs)
+                                         unsigned pb, long fs, bool keep = true)
 {
     constexpr int CI = SZ > 0 ? 1 : 3;
+    const unsigned fb = (unsigned)(fs * (long)sizeof(R)); // bytes of one field (< 4 GiB)
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         if (ST == 0) {
             const R f0 = w[c][CI];
-            stg_once(a->wout + c * fs, pb, f0 + a->alpha * hv[c]);
-            stg_once(a->tw + c * fs, pb, f0 + a->beta * hv[c]);
+            stg_keep(a->wout + c * fs, pb, f0 + a->alpha * hv[c], keep, fb);
+            stg_keep(a->tw + c * fs, pb, f0 + a->beta * hv[c], keep, fb);
         } else if (ST < 3) {
-            stg_once(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c]);
-            stg_once(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c]);
+            stg_keep(a->wout + c * fs, pb, in.fw[c] + a->alpha * hv[c], keep, fb);
+            stg_keep(a->tw + c * fs, pb, in.tw[c] + a->beta * hv[c], keep, fb);
         } else {
-            stg_once(a->wout + c * fs, pb, in.tw[c] + a->beta * hv[c]);
+            stg_keep(a->wout + c * fs, pb, in.tw[c] + a->beta * hv[c], keep, fb);
         }
     }
 }
@@ -712,16 +755,22 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     // so every load stays inside its own plane.
     const int yc = min(y, g.nyp - 1);
     // halo slots of this lane: one of the 4 halo rows (all 64 columns) for
-    // the first 4 wavefronts, and for lanes 0..4*TY-1 one of the 4 halo
-    // columns (the TY tile rows)
+    // the first 4 wavefronts, and for lanes 0..4*TY-1 of every wavefront one
+    // of the 4 halo columns (the TY tile rows) of the fields hk, hk+TY, ...
+    // (each wavefront stages the x halo of NH of the 9 fields)
     const int hk = tid >> 6;
     const int hrow = SX > 0 ? (hk == 0 ? 0 : TY + hk) : (hk < 3 ? hk : TY + 3);
     const int hy = y0 - O + hrow, hxx = x0 + tx;
     const bool hv1 = hk < 4 && hy >= -3 && hy < g.nyp;
-    const int hc2k = tid / TY, hr2 = tid % TY;
+    constexpr int NH = (NF + TY - 1) / TY;
+    const int hc2k = tx / TY, hr2 = tx % TY;
     const int hcol = SX > 0 ? (hc2k == 0 ? 0 : 64 + hc2k) : (hc2k < 3 ? hc2k : 67);
     const int hx2 = x0 - O + hcol, hy2 = y0 + hr2;
-    const bool hv2 = tid < 4 * TY && hx2 >= -3 && hx2 < g.nxp && hy2 < g.nyp;
+    const bool hv2 = tx < 4 * TY && hx2 >= -3 && hx2 < g.nxp && hy2 < g.nyp;
+    // field of x-halo slot j of this wavefront (wavefronts past the last
+    // field reload field 8 and discard it, so every wavefront issues NH loads)
+    auto hfield = [&](int j) { return min(hk + TY * j, NF - 1); };
+    auto hkeep = [&](int j) { return hk + TY * j < NF; };
     // byte offsets inside plane 0 (may be negative: added to a plane base);
     // invalid halo slots fall back to the lane's own cell
     const int rel_c = (yc * g.nxp + x) * (int)esz;
@@ -744,19 +793,23 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             const R v1 = ldg(a->win + c * fs0, pbase(p) + rel_1);
-            const R v2 = ldg(a->win + c * fs0, pbase(p) + rel_2);
             if (hk < 4) lds[slot(p)][c][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
-            if (tid < 4 * TY) lds[slot(p)][c][O + hr2][hcol] = (LT)(hv2 ? v2 : (R)0);
+        }
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            const R v2 = ldg(a->win + hfield(j) * fs0, pbase(p) + rel_2);
+            if (tx < 4 * TY && hkeep(j)) lds[slot(p)][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? v2 : (R)0);
         }
     }
 
-    R lead[NF], h1[NF], h2[NF];
+    R lead[NF], h1[NF], h2[NH];
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         lead[c] = ldg(a->win + c * fs0, pbase(z0 + D) + rel_c);
         h1[c] = ldg(a->win + c * fs0, pbase(z0 + HD) + rel_1);
-        h2[c] = ldg(a->win + c * fs0, pbase(z0 + HD) + rel_2);
     }
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h2[j] = ldg(a->win + hfield(j) * fs0, pbase(z0 + HD) + rel_2);
 
     for (int z = z0; z < z1; ++z) {
         // The field stride is laundered through an empty asm once per plane,
@@ -769,10 +822,10 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         asm volatile("" : "+s"(al));
         // Every lane loads its cell inputs and stores its results without a
         // branch: a lane outside the grid addresses its own zero guard cell
-        // (x in [ni, nxp) or a clamped guard row), reads zeros and stores
-        // zeros back.  With no skippable memory operation between them, the
-        // compiler's wait counts let this plane's cell loads stay in flight
-        // while the previous plane's prefetch is consumed.
+        // (x in [ni, nxp) or a clamped guard row) and reads zeros; its stores
+        // are dropped (rk_store).  With no skippable memory operation between
+        // them, the compiler's wait counts let this plane's cell loads stay in
+        // flight while the previous plane's prefetch is consumed.
         const unsigned pbz = pbase(z) + rel_c;
         CellIn<R> cin;
         load_cell<R, ST, HM>(al, pbz, fs, cin);
@@ -782,9 +835,11 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         for (int c = 0; c < NF; ++c) {
             lds[sc][c][O + ty][O + tx] = (LT)w[c][CI];
             if (hk < 4) lds[sh][c][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
-            if (tid < 4 * TY) lds[sh][c][O + hr2][hcol] = (LT)(hv2 ? h2[c] : (R)0);
             w[c][4] = lead[c];
         }
+#pragma unroll
+        for (int j = 0; j < NH; ++j)
+            if (tx < 4 * TY && hkeep(j)) lds[sh][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? h2[j] : (R)0);
         __syncthreads();
         // next plane's lead (plane z+1+D <= nk+3) and halo (plane z+1+HD <=
         // nk+3) lie inside the 4 zero planes above the grid
@@ -792,8 +847,9 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         for (int c = 0; c < NF; ++c) {
             lead[c] = ldg(al->win + c * fs, pbase(z + 1 + D) + rel_c);
             h1[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_1);
-            h2[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_2);
         }
+#pragma unroll
+        for (int j = 0; j < NH; ++j) h2[j] = ldg(al->win + hfield(j) * fs, pbase(z + 1 + HD) + rel_2);
         R hv[NF];
 #pragma unroll
         for (int c = 0; c < NF; ++c) hv[c] = (R)0;
@@ -815,8 +871,9 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
             }
             cell_rates<R, SX, SZ, ST>(al, k, cin, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
         }
-        // guard lanes: zero inputs and zero rates store zeros
-        rk_store<R, SZ, ST>(al, cin, w, hv, pbz, fs);
+        // lanes outside the grid: their buffer stores are dropped (FP32), or
+        // their zero inputs and zero rates store zeros (FP64)
+        rk_store<R, SZ, ST>(al, cin, w, hv, pbz, fs, active);
         if (active) pgv_update<R, ST>(al, k, cin, hv, z - (g.nk - 3));
 #pragma unroll
         for (int c = 0; c < NF; ++c) {

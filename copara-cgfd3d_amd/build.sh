@@ -22,6 +22,16 @@ echo "built $here/libcgfd3d.so"
 hipcc "${FLAGS[@]}" -shared "$here/csrc/fno_kernels.hip" -o "$here/libcgfd_fno.so.tmp"
 mv -f "$here/libcgfd_fno.so.tmp" "$here/libcgfd_fno.so"
 echo "built $here/libcgfd_fno.so"
+# the netCDF-4 (HDF5) input reader of the front end, where libhdf5 is installed
+HDF5_ROOT="${CGFD_HDF5_ROOT:-/opt/conda}"
+if [ -f "$HDF5_ROOT/include/hdf5.h" ]; then
+    gcc -O2 -fPIC -shared -Wall -I"$root/include" -I"$HDF5_ROOT/include" "$here/csrc/cgfd_nc4.c" \
+        -L"$HDF5_ROOT/lib" -lhdf5 -Wl,-rpath,"$HDF5_ROOT/lib" -o "$here/libcgfd_nc4.so.tmp"
+    mv -f "$here/libcgfd_nc4.so.tmp" "$here/libcgfd_nc4.so"
+    echo "built $here/libcgfd_nc4.so"
+else
+    echo "no hdf5.h under $HDF5_ROOT: libcgfd_nc4.so (netCDF-4 inputs) not built"
+fi
 # the C++ host driver (one child-rank run through the C-ABI)
 g++ -O2 -std=c++17 -Wall -I"$root/include" "$here/host/cgfd_child.cpp" -L"$here" -lcgfd3d \
     -Wl,-rpath,'$ORIGIN' -o "$here/cgfd_child"

@@ -49,6 +49,7 @@ SIGNATURES = {
     "cgfd_snap_shape": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP]),
     "cgfd_get_snap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
     "cgfd_set_pgv": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_set_hybrid": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_init": (ctypes.c_int, [_VP]),
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -304,6 +305,11 @@ class Simulation:
     def set_pgv(self, on=True):
         self._check(self.lib.cgfd_set_pgv(self.h, 1 if on else 0), "set_pgv")
 
+    def set_hybrid(self, conindex):
+        """hybrid_grid = 1: the reference's ConIndex (z index with the 3 ghost
+        planes); below it the Cartesian contraction applies."""
+        self._check(self.lib.cgfd_set_hybrid(self.h, int(conindex)), "set_hybrid")
+
     def init(self):
         self._check(self.lib.cgfd_init(self.h), "init")
 
@@ -323,6 +329,8 @@ class Simulation:
         if sc.get("snapshots") is not None and len(sc["snapshots"]):
             self.set_snapshots(sc["snapshots"])
         self.set_pgv(sc.get("pgv", True))
+        if sc.get("conindex") is not None:
+            self.set_hybrid(sc["conindex"])
         self.init()
 
     # ---- stepping ---------------------------------------------------------

@@ -58,6 +58,7 @@ struct cgfd_ctx {
     bool have_metrics = false, have_media = false;
     bool hmet = false;     // xi/eta metrics are the constants 1/0 in every valid cell
     bool use_hmet = true;  // let the stage kernel exploit it
+    int zcon = 0;          // hybrid grid: engine planes z < zcon use the Cartesian contraction
 
     // point sources: host copies until init builds the device tables
     int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
@@ -630,6 +631,18 @@ int cgfd_snap_shape(const cgfd_ctx *c, int isnap, int *npoints, int *ntime)
     return 0;
 }
 
+int cgfd_set_hybrid(cgfd_ctx *c, int conindex)
+{
+    if (!c) return CGFD_EINVAL;
+    if (conindex < 3 || conindex > c->g.nk)
+        return fail(c, CGFD_EINVAL,
+                    "conindex must lie in [3, nk]: the curvilinear contraction applies from that z index (with "
+                    "the 3 ghost planes) up, and the 3 surface rows must stay curvilinear");
+    c->zcon = conindex - 3;
+    c->args_dirty = true;
+    return 0;
+}
+
 int cgfd_set_pgv(cgfd_ctx *c, int on)
 {
     if (!c) return CGFD_EINVAL;
@@ -863,6 +876,7 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     sa.beta = RK4B[k] * dt;
     sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
     sa.hmet = (c->hmet && c->use_hmet) ? 1 : 0;
+    sa.zcon = c->zcon;
     sa.step = c->d_step;
     return sa;
 }

@@ -41,6 +41,7 @@ template <typename R> struct StageArgs {
     R alpha, beta;
     int zlen;
     int hmet; // 1: horizontal metrics are the constants 1/0 (variant 0 loads only the zeta metrics)
+    int zcon; // hybrid grid: planes z < zcon take the Cartesian contraction (0: none)
     const int *step; // device step counter: the current step currT (Tindex = 2*currT + tinc)
 };
 

@@ -300,34 +300,72 @@ __device__ __forceinline__ void cell_rates(AP a, const Column<R> &k, const CellI
     const R etx = in.m[ETX], ety = in.m[ETY], etz = in.m[ETZ];
     const R ztx = in.m[ZTX], zty = in.m[ZTY], ztz = in.m[ZTZ];
 
-    const R DxiVx = (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ]) * rho;
-    const R DetVx = (etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ]) * rho;
-    const R DztVx = (ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]) * rho;
-    const R DxiVy = (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ]) * rho;
-    const R DetVy = (etx * Dy[TXY] + ety * Dy[TYY] + etz * Dy[TYZ]) * rho;
-    const R DztVy = (ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]) * rho;
-    const R DxiVz = (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ]) * rho;
-    const R DetVz = (etx * Dy[TXZ] + ety * Dy[TYZ] + etz * Dy[TZZ]) * rho;
-    const R DztVz = (ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]) * rho;
+    // Contraction with the metrics.  With the hybrid grid (hybrid_grid = 1),
+    // planes below the conversion depth (z < zcon, uniform over the
+    // workgroup) take CalWave's Cartesian form, which uses only xi_x, eta_y
+    // and zeta_z in its own operand order (calculate.cu:3910-3956);
+    // elsewhere the curvilinear contraction of CalWaveCL (4855-4930).
+    R DxiVx, DetVx, DztVx, DxiVy, DetVy, DztVy, DxiVz, DetVz, DztVz;
+    R DxiTxx, DetTxx, DztTxx, DxiTyy, DetTyy, DztTyy, DxiTzz, DetTzz, DztTzz;
+    R DxiTxy, DetTxy, DztTxy, DxiTxz, DetTxz, DztTxz, DxiTyz, DetTyz, DztTyz;
+    if (z >= a->zcon) {
+        DxiVx = (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ]) * rho;
+        DetVx = (etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ]) * rho;
+        DztVx = (ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]) * rho;
+        DxiVy = (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ]) * rho;
+        DetVy = (etx * Dy[TXY] + ety * Dy[TYY] + etz * Dy[TYZ]) * rho;
+        DztVy = (ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]) * rho;
+        DxiVz = (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ]) * rho;
+        DetVz = (etx * Dy[TXZ] + ety * Dy[TYZ] + etz * Dy[TZZ]) * rho;
+        DztVz = (ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]) * rho;
 
-    const R DxiTxx = lam2mu * xix * Dx[VX] + lambda * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
-    const R DetTxx = lam2mu * etx * Dy[VX] + lambda * ety * Dy[VY] + lambda * etz * Dy[VZ];
-    const R DztTxx = lam2mu * ztx * Dz[VX] + lambda * zty * Dz[VY] + lambda * ztz * Dz[VZ];
-    const R DxiTyy = lambda * xix * Dx[VX] + lam2mu * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
-    const R DetTyy = lambda * etx * Dy[VX] + lam2mu * ety * Dy[VY] + lambda * etz * Dy[VZ];
-    const R DztTyy = lambda * ztx * Dz[VX] + lam2mu * zty * Dz[VY] + lambda * ztz * Dz[VZ];
-    const R DxiTzz = lambda * xix * Dx[VX] + lambda * xiy * Dx[VY] + lam2mu * xiz * Dx[VZ];
-    const R DetTzz = lambda * etx * Dy[VX] + lambda * ety * Dy[VY] + lam2mu * etz * Dy[VZ];
-    const R DztTzz = lambda * ztx * Dz[VX] + lambda * zty * Dz[VY] + lam2mu * ztz * Dz[VZ];
-    const R DxiTxy = (xiy * Dx[VX] + xix * Dx[VY]) * miu;
-    const R DetTxy = (ety * Dy[VX] + etx * Dy[VY]) * miu;
-    const R DztTxy = (zty * Dz[VX] + ztx * Dz[VY]) * miu;
-    const R DxiTxz = (xiz * Dx[VX] + xix * Dx[VZ]) * miu;
-    const R DetTxz = (etz * Dy[VX] + etx * Dy[VZ]) * miu;
-    const R DztTxz = (ztz * Dz[VX] + ztx * Dz[VZ]) * miu;
-    const R DxiTyz = (xiz * Dx[VY] + xiy * Dx[VZ]) * miu;
-    const R DetTyz = (etz * Dy[VY] + ety * Dy[VZ]) * miu;
-    const R DztTyz = (ztz * Dz[VY] + zty * Dz[VZ]) * miu;
+        DxiTxx = lam2mu * xix * Dx[VX] + lambda * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
+        DetTxx = lam2mu * etx * Dy[VX] + lambda * ety * Dy[VY] + lambda * etz * Dy[VZ];
+        DztTxx = lam2mu * ztx * Dz[VX] + lambda * zty * Dz[VY] + lambda * ztz * Dz[VZ];
+        DxiTyy = lambda * xix * Dx[VX] + lam2mu * xiy * Dx[VY] + lambda * xiz * Dx[VZ];
+        DetTyy = lambda * etx * Dy[VX] + lam2mu * ety * Dy[VY] + lambda * etz * Dy[VZ];
+        DztTyy = lambda * ztx * Dz[VX] + lam2mu * zty * Dz[VY] + lambda * ztz * Dz[VZ];
+        DxiTzz = lambda * xix * Dx[VX] + lambda * xiy * Dx[VY] + lam2mu * xiz * Dx[VZ];
+        DetTzz = lambda * etx * Dy[VX] + lambda * ety * Dy[VY] + lam2mu * etz * Dy[VZ];
+        DztTzz = lambda * ztx * Dz[VX] + lambda * zty * Dz[VY] + lam2mu * ztz * Dz[VZ];
+        DxiTxy = (xiy * Dx[VX] + xix * Dx[VY]) * miu;
+        DetTxy = (ety * Dy[VX] + etx * Dy[VY]) * miu;
+        DztTxy = (zty * Dz[VX] + ztx * Dz[VY]) * miu;
+        DxiTxz = (xiz * Dx[VX] + xix * Dx[VZ]) * miu;
+        DetTxz = (etz * Dy[VX] + etx * Dy[VZ]) * miu;
+        DztTxz = (ztz * Dz[VX] + ztx * Dz[VZ]) * miu;
+        DxiTyz = (xiz * Dx[VY] + xiy * Dx[VZ]) * miu;
+        DetTyz = (etz * Dy[VY] + ety * Dy[VZ]) * miu;
+        DztTyz = (ztz * Dz[VY] + zty * Dz[VZ]) * miu;
+    } else {
+        DxiVx = rho * xix * Dx[TXX];
+        DetVx = rho * ety * Dy[TXY];
+        DztVx = ztz * Dz[TXZ] * rho;
+        DxiVy = rho * xix * Dx[TXY];
+        DetVy = rho * ety * Dy[TYY];
+        DztVy = ztz * Dz[TYZ] * rho;
+        DxiVz = rho * xix * Dx[TXZ];
+        DetVz = rho * ety * Dy[TYZ];
+        DztVz = ztz * Dz[TZZ] * rho;
+        DxiTxx = lam2mu * xix * Dx[VX];
+        DetTxx = lambda * ety * Dy[VY];
+        DztTxx = lambda * ztz * Dz[VZ];
+        DxiTyy = lambda * xix * Dx[VX];
+        DetTyy = lam2mu * ety * Dy[VY];
+        DztTyy = lambda * ztz * Dz[VZ];
+        DxiTzz = lambda * xix * Dx[VX];
+        DetTzz = lambda * ety * Dy[VY];
+        DztTzz = lam2mu * ztz * Dz[VZ];
+        DxiTxy = miu * xix * Dx[VY];
+        DetTxy = miu * ety * Dy[VX];
+        DztTxy = (R)0;
+        DxiTxz = miu * xix * Dx[VZ];
+        DetTxz = (R)0;
+        DztTxz = ztz * Dz[VX] * miu;
+        DxiTyz = (R)0;
+        DetTyz = miu * ety * Dy[VZ];
+        DztTyz = ztz * Dz[VY] * miu;
+    }
 
     const R *pmz = a->pml + 3 * g.ni + 3 * g.nj;
     const R APDz = pmz[z], Bz = pmz[g.nk + z], DBz = pmz[2 * g.nk + z];
@@ -1071,6 +1109,17 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
                 // the xi-derivative of component TXX+i
                 const R rho = med0;
                 constexpr int XX = 0, YY = 1, ZZ = 2, XY = 3, XZ = 4, YZ = 5;
+                if (z < a->zcon) { // hybrid grid below the conversion depth (calculate.cu:3910-3925)
+                    Dxi[0] = rho * xix * Dx[XX];
+                    Det[0] = rho * ety * Dy[XY];
+                    Dzt[0] = ztz * Dz[XZ] * rho;
+                    Dxi[1] = rho * xix * Dx[XY];
+                    Det[1] = rho * ety * Dy[YY];
+                    Dzt[1] = ztz * Dz[YZ] * rho;
+                    Dxi[2] = rho * xix * Dx[XZ];
+                    Det[2] = rho * ety * Dy[YZ];
+                    Dzt[2] = ztz * Dz[ZZ] * rho;
+                } else {
                 Dxi[0] = (xix * Dx[XX] + xiy * Dx[XY] + xiz * Dx[XZ]) * rho;
                 Det[0] = (etx * Dy[XX] + ety * Dy[XY] + etz * Dy[XZ]) * rho;
                 Dzt[0] = (ztx * Dz[XX] + zty * Dz[XY] + ztz * Dz[XZ]) * rho;
@@ -1080,6 +1129,7 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
                 Dxi[2] = (xix * Dx[XZ] + xiy * Dx[YZ] + xiz * Dx[ZZ]) * rho;
                 Det[2] = (etx * Dy[XZ] + ety * Dy[YZ] + etz * Dy[ZZ]) * rho;
                 Dzt[2] = (ztx * Dz[XZ] + zty * Dz[YZ] + ztz * Dz[ZZ]) * rho;
+                }
             } else {
                 // velocity zeta-derivatives of the top three rows (CondFreeVLOW,
                 // calculate.cu:4736-4769)
@@ -1117,6 +1167,26 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
                 lam2mu = med0;
                 miu = med1;
                 lambda = lam2mu - (R)2 * miu;
+                if (z < a->zcon) { // hybrid grid below the conversion depth (calculate.cu:3927-3956)
+                    Dxi[0] = lam2mu * xix * Dx[0];
+                    Det[0] = lambda * ety * Dy[1];
+                    Dzt[0] = lambda * ztz * Dz[2];
+                    Dxi[1] = lambda * xix * Dx[0];
+                    Det[1] = lam2mu * ety * Dy[1];
+                    Dzt[1] = lambda * ztz * Dz[2];
+                    Dxi[2] = lambda * xix * Dx[0];
+                    Det[2] = lambda * ety * Dy[1];
+                    Dzt[2] = lam2mu * ztz * Dz[2];
+                    Dxi[3] = miu * xix * Dx[1];
+                    Det[3] = miu * ety * Dy[0];
+                    Dzt[3] = (R)0;
+                    Dxi[4] = miu * xix * Dx[2];
+                    Det[4] = (R)0;
+                    Dzt[4] = ztz * Dz[0] * miu;
+                    Dxi[5] = (R)0;
+                    Det[5] = miu * ety * Dy[2];
+                    Dzt[5] = ztz * Dz[1] * miu;
+                } else {
                 Dxi[0] = lam2mu * xix * Dx[0] + lambda * xiy * Dx[1] + lambda * xiz * Dx[2];
                 Det[0] = lam2mu * etx * Dy[0] + lambda * ety * Dy[1] + lambda * etz * Dy[2];
                 Dzt[0] = lam2mu * ztx * Dz[0] + lambda * zty * Dz[1] + lambda * ztz * Dz[2];
@@ -1135,6 +1205,7 @@ stage_split_kernel(const StageArgs<R> *pa, int tinc)
                 Dxi[5] = (xiz * Dx[1] + xiy * Dx[2]) * miu;
                 Det[5] = (etz * Dy[1] + ety * Dy[2]) * miu;
                 Dzt[5] = (ztz * Dz[1] + zty * Dz[2]) * miu;
+                }
             }
 #pragma unroll
             for (int c = 0; c < NO; ++c)

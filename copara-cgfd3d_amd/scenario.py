@@ -287,7 +287,7 @@ def von_karman_field(shape, dh, a, nu, rng, dtype=F32):
         k2 = k2 + (k.reshape(sh) ** 2).astype(np.float32)
     F *= ((1.0 + k2 * a * a) ** (-(nu + d / 2.0) / 2.0)).astype(np.float32)
     del k2
-    f = np.fft.irfftn(F, s=shape).astype(dtype)
+    f = np.fft.irfftn(F, s=shape, axes=tuple(range(len(shape)))).astype(dtype)
     del F
     f -= f.mean()
     f /= max(float(np.sqrt(np.mean(f.astype(np.float64) ** 2))), 1e-30)

@@ -20,12 +20,12 @@ Supported inputs and the reference paths they follow:
 
 | input | reference |
 |---|---|
-| grid_type vmap, hybrid_grid 0 | gridmesh.cpp:72-532, 1092-1210, 1446-1470 |
+| grid_type vmap, hybrid_grid 0 and 1 (conversion_interface_depth) | gridmesh.cpp:72-532, 1092-1210, 1446-1470 |
 | absorb file (ADE CFS-PML) | absorb.cpp:20-46 |
 | source_type 2/3: force and moment anchors, 8 STF types, moment or angle mechanisms | source.cpp:12-155, 671-849, 985-1289 |
-| source_type 1: focal file in netCDF classic format | source.cpp:43-58, 787-848, 1185-1262 |
+| source_type 1: focal file, netCDF classic or netCDF-4 | source.cpp:43-58, 787-848, 1185-1262 |
 | station file: snapshots, receivers, lines | seisplot.cpp:12-55, 330-457 |
-| media_type volume in netCDF classic format | mediaparas.cpp:376-441, 1273-1425, 2140-2219 |
+| media_type volume, netCDF classic or netCDF-4 | mediaparas.cpp:376-441, 1273-1425, 2140-2219 |
 | media_type interface3D | mediaparas.cpp:34-43, 52-374, 442-914, 1837-1905 |
 | sampling_point_per_cell > 1 (media smoothing) | mediaparas.cpp:1712-1835 |
 
@@ -35,9 +35,6 @@ Refused, with the reason in the error:
 - media_type interface: `readdata` overwrites the media from `read_interface`
   with zero Vp/Vs/rho via `get2in1` (mediaparas.cpp:2163-2200), and
   `MediaStatistics` then sets every cell to sqrt(0/0).
-- hybrid_grid 1: needs the hybrid Cartesian kernels, which are not part of
-  this engine.
-- NETCDF4/HDF5 media and focal files: no HDF5 reader is available here.
 
 Callers may always pass their own `media=(alpha, beta, rho)` arrays instead.
 """
@@ -187,6 +184,7 @@ def read_main(conf_path, seispath=None):
         "stept": c.real("stept"), "steph": c.real("steph"),
         "seispath": seispath if seispath is not None else c.str("seispath"),
         "grid_type": c.str("grid_type"), "hybrid_grid": c.int("hybrid_grid"),
+        "conversion_depth": c.real("conversion_interface_depth"),
         "media_type": c.str("media_type"),
         "sampling": [c.int("sampling_point_per_cell", 3 + i) for i in range(3)],
         "crit_value": c.real("threshold_velocity_jump_of_interface"),
@@ -328,10 +326,79 @@ def read_vmap(path, ni, nj, nk):
     return posx, posy, posz, gp, eq
 
 
-def vmap_coords(posx, posy, posz, gp, eq, nk):
+def hybrid_layers(posz, gp, eq, nk, conv_depth):
+    """The hybrid grid's z layering (gridmesh.cpp:243-375): the layer that
+    holds the conversion depth ends there, and one more layer runs flat from
+    it down to the shallowest point of the last interface; every layer's
+    share of the nk grid planes is recomputed from its thickness.  Returns
+    (Zdep [ni][nj][L+1], GPs [L], SpaceEQ [L], indexZ [L+1], ConIndex) for
+    L = ConInterface + 1 layers; the planes below ConIndex lie on flat
+    equally spaced levels."""
+    ni, nj, ns = posz.shape
+    gint = [(F32(posz[:, :, k].min()), F32(posz[:, :, k].max())) for k in range(ns)]
+    glay = [(F32(posz[:, :, k + 1].min()), F32(posz[:, :, k].max())) for k in range(ns - 1)]
+    cd = F32(conv_depth)
+    con = None
+    for k in range(ns - 1):
+        if glay[k][0] <= cd <= glay[k][1]:
+            con = k + 1
+            break
+    if con is None:
+        raise ConfError(f"conversion_interface_depth {float(cd)} m lies in no layer of the vmap grid")
+    top, bottom = cd, gint[ns - 1][0]
+    space = [bool(eq[k]) for k in range(con)] + [True]
+    zdep = np.empty((ni, nj, con + 2), F32)
+    zdep[:, :, :con] = posz[:, :, :con]
+    zdep[:, :, con] = top
+    zdep[:, :, con + 1] = bottom
+    layer = [list(glay[k]) for k in range(con)] + [[bottom, top]]
+    layer[con - 1][0] = top
+    total = F32(0.0)
+    for k in range(con + 1):
+        total = F32(total + F32(layer[k][1] - layer[k][0]))
+    gps = [0] * (con + 1)
+    gps[con] = nk - 1
+    for k in range(con):
+        part = F32(layer[k][1] - layer[k][0])
+        gps[k] = int(F32(F32(F32(nk) * part) / total))    # int = int*Real/Real
+        gps[con] -= gps[k]
+    index = [0] * (con + 2)
+    index[con + 1] = LFD
+    for k in range(con, -1, -1):
+        index[k] = index[k + 1] + gps[k]
+    return zdep, gps, space, index, index[con + 1] + gps[con]
+
+
+def _fill_z(zdep, gps, space, index, nk):
+    """Column depths zvec (ni, nj, nk+6) between the interfaces zdep at the
+    plane indices index (gridmesh.cpp:395-414, 469-496): equally spaced
+    layers by the float expression of the source, transition layers by
+    smooth_zdepth."""
+    ni, nj, _ = zdep.shape
+    zvec = np.zeros((ni, nj, nk + 2 * LFD), F32)
+    for k in range(len(gps)):
+        hi, lo = index[k], index[k + 1]
+        if space[k]:
+            step = ((zdep[:, :, k] - zdep[:, :, k + 1]).astype(np.float64) / (1.0 * gps[k])).astype(F32)
+            for m in range(lo, hi + 1):
+                zvec[:, :, m] = zdep[:, :, k] + F32(m - hi) * step
+        else:
+            for i in range(ni):
+                for j in range(nj):
+                    p = zdep[i, j]
+                    zs0 = F32(float(F32(p[k - 1] - p[k])) / (1.0 * gps[k - 1]))
+                    zs1 = F32(float(F32(p[k + 1] - p[k + 2])) / (1.0 * gps[k + 1]))
+                    zvec[i, j, lo:lo + gps[k] + 1] = smooth_zdepth(p[k], p[k + 1], zs0, zs1, gps[k])
+        zvec[:, :, hi] = zdep[:, :, k]
+        zvec[:, :, lo] = zdep[:, :, k + 1]
+    return zvec
+
+
+def vmap_coords(posx, posy, posz, gp, eq, nk, conv_depth=None):
     """Coordinates with ghosts (x, y, z each (ni+6, nj+6, nk+6)) from vmap
-    anchors: the all-curvilinear branch of read_vmapnew
-    (gridmesh.cpp:439-513) and the ghost extrapolation of readdata
+    anchors, and ConIndex: the all-curvilinear branch of read_vmapnew
+    (gridmesh.cpp:439-513, ConIndex = nk1) or, with a conversion depth, its
+    hybrid branch (243-431); then the ghost extrapolation of readdata
     (gridmesh.cpp:1467-1469)."""
     ni, nj, ns = posz.shape
     shape = (ni + 2 * LFD, nj + 2 * LFD, nk + 2 * LFD)
@@ -341,30 +408,20 @@ def vmap_coords(posx, posy, posz, gp, eq, nk):
     v = (slice(LFD, LFD + ni), slice(LFD, LFD + nj))
     x[v] = posx[:, :, None]
     y[v] = posy[:, :, None]
-    zindex = [0] * ns
-    zindex[ns - 1] = LFD
-    for i in range(ns - 2, -1, -1):
-        zindex[i] = zindex[i + 1] + gp[i]
-    zvec = np.zeros((ni, nj, nk + 2 * LFD), F32)
-    for k in range(ns - 1):
-        hi, lo = zindex[k], zindex[k + 1]
-        if eq[k]:
-            step = ((posz[:, :, k] - posz[:, :, k + 1]).astype(np.float64) / (1.0 * gp[k])).astype(F32)
-            for m in range(lo, hi + 1):
-                zvec[:, :, m] = posz[:, :, k] + F32(m - hi) * step
-        else:
-            for i in range(ni):
-                for j in range(nj):
-                    p = posz[i, j]
-                    zs0 = F32(float(F32(p[k - 1] - p[k])) / (1.0 * gp[k - 1]))
-                    zs1 = F32(float(F32(p[k + 1] - p[k + 2])) / (1.0 * gp[k + 1]))
-                    zvec[i, j, lo:lo + gp[k] + 1] = smooth_zdepth(p[k], p[k + 1], zs0, zs1, gp[k])
-        zvec[:, :, hi] = posz[:, :, k]
-        zvec[:, :, lo] = posz[:, :, k + 1]
+    if conv_depth is None:
+        zindex = [0] * ns
+        zindex[ns - 1] = LFD
+        for i in range(ns - 2, -1, -1):
+            zindex[i] = zindex[i + 1] + gp[i]
+        zvec = _fill_z(posz, list(gp), [bool(e) for e in eq], zindex, nk)
+        conindex = LFD
+    else:
+        zdep, gps, space, index, conindex = hybrid_layers(posz, gp, eq, nk, conv_depth)
+        zvec = _fill_z(zdep, gps, space, index, nk)
     z[v + (slice(LFD, LFD + nk),)] = zvec[:, :, LFD:LFD + nk]
     for a in (x, y, z):
         _sc.interpolated_extend(a)
-    return x, y, z
+    return x, y, z, conindex
 
 
 # ---------------------------------------------------------------------------
@@ -499,17 +556,117 @@ def _inside(loc, ni, nj, nk):
 # ---------------------------------------------------------------------------
 # sources (source.cpp:12-70, 671-849, 985-1289)
 # ---------------------------------------------------------------------------
-def _read_netcdf_classic(path, what):
+_NC4 = None
+
+
+def _nc4_library():
+    """libcgfd_nc4.so (built where libhdf5 is installed), or None."""
+    global _NC4
+    if _NC4 is None:
+        path = os.environ.get("CGFD_NC4_LIB", os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                          "libcgfd_nc4.so"))
+        try:
+            lib = ctypes.CDLL(path)
+        except OSError:
+            _NC4 = False
+            return None
+        C, LL, VP, IP = ctypes.c_char_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+        lib.cgfd_nc4_probe.argtypes = [C]
+        lib.cgfd_nc4_var_dims.argtypes = [C, C, IP, ctypes.POINTER(LL)]
+        lib.cgfd_nc4_get_var_float.argtypes = [C, C, VP, LL]
+        lib.cgfd_nc4_get_att_float.argtypes = [C, C, ctypes.POINTER(ctypes.c_float)]
+        for fn in ("cgfd_nc4_probe", "cgfd_nc4_var_dims", "cgfd_nc4_get_var_float", "cgfd_nc4_get_att_float"):
+            getattr(lib, fn).restype = ctypes.c_int
+        _NC4 = lib
+    return _NC4 or None
+
+
+class _NC4File:
+    """The parts of a netCDF file the front end reads, from a netCDF-4
+    (HDF5) file: dimension lengths, float variables and float global
+    attributes, as nc_inq_dimlen, nc_get_var_float and nc_get_att_float
+    return them (seisnc.cpp:1148-1228)."""
+
+    def __init__(self, lib, path, what):
+        self._lib, self._path, self._what = lib, os.fsencode(path), what
+        self.dimensions = _NC4Dims(self)
+        self.variables = _NC4Vars(self)
+
+    def _shape(self, name):
+        r = ctypes.c_int(0)
+        d = (ctypes.c_longlong * 8)()
+        if self._lib.cgfd_nc4_var_dims(self._path, name.encode(), ctypes.byref(r), d) != 0:
+            raise ConfError(f"{self._what} {self._path.decode()} has no variable or dimension {name}")
+        return tuple(int(d[i]) for i in range(r.value))
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        v = ctypes.c_float(0)
+        if self._lib.cgfd_nc4_get_att_float(self._path, name.encode(), ctypes.byref(v)) != 0:
+            raise AttributeError(f"{self._what} has no global attribute {name}")
+        return F32(v.value)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+class _NC4Dims:
+    def __init__(self, f):
+        self._f = f
+
+    def __getitem__(self, name):
+        shape = self._f._shape(name)
+        return shape[0] if shape else 1
+
+
+class _NC4Var:
+    def __init__(self, f, name):
+        self._f, self._name = f, name
+
+    def __getitem__(self, key):
+        shape = self._f._shape(self._name)
+        out = np.empty(shape, F32)
+        if self._f._lib.cgfd_nc4_get_var_float(self._f._path, self._name.encode(),
+                                               out.ctypes.data_as(ctypes.c_void_p), out.size) != 0:
+            raise ConfError(f"fail to read variable {self._name} of {self._f._what}")
+        return out[key]
+
+
+class _NC4Vars:
+    def __init__(self, f):
+        self._f = f
+
+    def __getitem__(self, name):
+        self._f._shape(name)
+        return _NC4Var(self._f, name)
+
+
+def _read_netcdf(path, what):
+    """A netCDF input file: the classic format through scipy's reader, the
+    netCDF-4 format (HDF5) through libcgfd_nc4.so and libhdf5."""
     from scipy.io import netcdf_file
 
     try:
         with open(path, "rb") as f:
-            magic = f.read(4)
+            magic = f.read(8)
     except OSError as e:
         raise ConfError(f"fail to open {what} {path}: {e.strerror}") from None
-    if magic[:3] != b"CDF":
-        raise ConfError(f"{what} {path} is not a netCDF classic file (NETCDF4/HDF5 files cannot be read here)")
-    return netcdf_file(path, "r", mmap=False)
+    if magic[:3] == b"CDF":
+        return netcdf_file(path, "r", mmap=False)
+    if magic == b"\x89HDF\r\n\x1a\n":
+        lib = _nc4_library()
+        if lib is None or lib.cgfd_nc4_probe(os.fsencode(path)) != 1:
+            if lib is None:
+                raise ConfError(f"{what} {path} is a netCDF-4 (HDF5) file, and the netCDF-4 reader "
+                                f"(libcgfd_nc4.so, built against libhdf5) is not available; convert it to the "
+                                f"classic format")
+            raise ConfError(f"{what} {path} has the HDF5 signature but does not open as an HDF5 file")
+        return _NC4File(lib, path, what)
+    raise ConfError(f"{what} {path} is neither a netCDF classic nor a netCDF-4 file")
 
 
 def read_sources(path, seispath, nt, stept):
@@ -523,7 +680,7 @@ def read_sources(path, seispath, nt, stept):
     fd0 = c.real("simulate_damp")  # read for every source type (source.cpp:786-788)
     if stype == 1:
         fn = f"{seispath}/{c.str('focal_process_NC_file')}"
-        with _read_netcdf_classic(fn, "focal process file") as f:
+        with _read_netcdf(fn, "focal process file") as f:
             npnt, fnt = f.dimensions["NP"], f.dimensions["NT"]
             fdt = F32(f.DT if np.ndim(f.DT) == 0 else f.DT[0])
             ser = np.stack([np.asarray(f.variables[nm][:], F32).reshape(npnt, fnt) * fd0
@@ -688,7 +845,7 @@ def _volume_table(path, x, y, z, zwmedia):
     is built with ZWmedia, as its Makefile does -- and Vp, Vs, rho on
     [x][y][z]) cut to the grid's coordinate range."""
     zname = "depth" if zwmedia else "z"
-    with _read_netcdf_classic(path, "media volume file") as f:
+    with _read_netcdf(path, "media volume file") as f:
         X = np.asarray(f.variables["x"][:], F32)
         Y = np.asarray(f.variables["y"][:], F32)
         Z = np.asarray(f.variables[zname][:], F32)
@@ -1078,11 +1235,9 @@ def load(conf_path, seispath=None, media=None, zwmedia=True):
                         "(gridmesh.cpp:52-68), so it yields no valid grid; use a vmap grid")
     if m["grid_type"] != "vmap":
         raise ConfError(f"gridtype configure {m['grid_type']} is wrong when reading grid data")
-    if m["hybrid_grid"] != 0:
-        raise ConfError("hybrid_grid 1 needs the hybrid Cartesian kernels, which this engine does not have; "
-                        "set hybrid_grid = 0 for the all-curvilinear method")
     posx, posy, posz, gp, eq = read_vmap(m["grid_filename"], ni, nj, nk)
-    x, y, z = vmap_coords(posx, posy, posz, gp, eq, nk)
+    x, y, z, conindex = vmap_coords(posx, posy, posz, gp, eq, nk,
+                                    m["conversion_depth"] if m["hybrid_grid"] else None)
     met = _sc.metrics_from_coords(x, y, z, m["steph"])
 
     smooth = (*m["sampling"], m["crit_value"], m["crit_perc"])
@@ -1108,6 +1263,8 @@ def load(conf_path, seispath=None, media=None, zwmedia=True):
     sc = dict(ni=ni, nj=nj, nk=nk, nt=nt, dh=float(m["steph"]), dt=float(m["stept"]), metrics=met,
               alpha=alpha, beta=beta, rho=rho, nabs=nabs, pml=pml, pgv=bool(m["output_peakvel"]),
               coords=(x, y, z), name=f"conf-{os.path.basename(os.path.dirname(os.path.abspath(conf_path)))}")
+    if m["hybrid_grid"]:
+        sc["conindex"] = conindex
     sc.update(locate_sources(src, x, y, z, rho, beta))
     st = read_stations(m["station_filename"])
     sc["recv_locs"] = locate_stations(st, x, y, z)

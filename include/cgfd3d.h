@@ -108,6 +108,15 @@ int cgfd_snap_shape(const cgfd_ctx *ctx, int isnap, int *npoints, int *ntime);
 int cgfd_get_snap(cgfd_ctx *ctx, int isnap, int comp, void *out);
 /* output_peakvel (PVF, calculate.cu:134): running max |V| on the surface. */
 int cgfd_set_pgv(cgfd_ctx *ctx, int on);
+/* hybrid_grid = 1: the ChildProcs members HyGrid/ConIndex (typedata.h:143),
+ * which select CalDiff/CalWave instead of CalDiffCL/CalWaveCL
+ * (calculate.cu:1469-1505).  conindex is the reference's ConIndex, a z index
+ * counted with the 3 ghost planes (gridmesh.cpp:346): below it the rates use
+ * CalWave's Cartesian contraction of xi_x, eta_y and zeta_z
+ * (calculate.cu:3910-3956), from it up the curvilinear one.  The default,
+ * 3 (= nk1), is the all-curvilinear grid of hybrid_grid = 0.  CGFD_EINVAL
+ * unless 3 <= conindex <= nk (the three surface rows stay curvilinear). */
+int cgfd_set_hybrid(cgfd_ctx *ctx, int conindex);
 
 /* VelCoeff (calculate.cu:1336-1353) plus state reset: zero the wavefield,
  * auxiliary fields, traces and PGV; compute the traction-free closure

@@ -82,6 +82,7 @@ struct orc {
     real *seis;
     int pgv_on;
     real *pgv[3];
+    int conindex; /* ConIndex: the curvilinear contraction applies at idz >= conindex */
 };
 
 static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
@@ -148,6 +149,7 @@ orc_t *orc_create(int ni, int nj, int nk, real dh, real dt, int nt)
     o->NX = ni + 2 * LFD; o->NY = nj + 2 * LFD; o->NZ = nk + 2 * LFD;
     o->N = (long)o->NX * o->NY * o->NZ;
     o->dh = dh; o->dt = dt; o->nt = nt;
+    o->conindex = LFD; /* hybrid_grid = 0: ConIndex = nk1 (gridmesh.cpp:441, simulation.cc:113) */
     for (int k = 0; k < NDRV; k++) o->drv[k] = (real *)zalloc(o->N * sizeof(real));
     o->alpha = (real *)zalloc(o->N * sizeof(real));
     o->beta = (real *)zalloc(o->N * sizeof(real));
@@ -304,6 +306,15 @@ int orc_set_receivers(orc_t *o, int n, const int *locs)
 }
 
 int orc_set_pgv(orc_t *o, int on) { o->pgv_on = on ? 1 : 0; return 0; }
+
+/* hybrid_grid = 1: ConIndex from the hybrid mesh (gridmesh.cpp:346); the
+ * three surface rows stay curvilinear */
+int orc_set_conindex(orc_t *o, int conindex)
+{
+    if (conindex < LFD || conindex > o->nk + LFD - 3) return -1;
+    o->conindex = conindex;
+    return 0;
+}
 
 /* ---- VelPDcoeff, calculate.cu:3320-3466 --------------------------------- */
 static void matinv3(real *A) /* calculate.cu:3444-3466, Gauss-Jordan without pivoting */
@@ -481,46 +492,93 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
                 real lam2mu = rho * o->alpha[G] * o->alpha[G];
                 real lambda = lam2mu - 2.0 * miu;
                 rho = 1.0 / rho;
-                real xix = o->drv[XIX][G], xiy = o->drv[XIY][G], xiz = o->drv[XIZ][G];
-                real etx = o->drv[ETX][G], ety = o->drv[ETY][G], etz = o->drv[ETZ][G];
-                real ztx = o->drv[ZTX][G], zty = o->drv[ZTY][G], ztz = o->drv[ZTZ][G];
+                /* CalWave (hybrid grid) below the conversion depth loads only xi_x,
+                 * eta_y and zeta_z and uses a Cartesian form of the contraction
+                 * (calculate.cu:3828-3956); at and above ConIndex, and everywhere
+                 * without the hybrid grid (ConIndex = nk1), the curvilinear
+                 * contraction of CalWaveCL (calculate.cu:4855-4930) */
+                real xix = o->drv[XIX][G], ety = o->drv[ETY][G], ztz = o->drv[ZTZ][G];
+                real xiy = 0, xiz = 0, etx = 0, etz = 0, ztx = 0, zty = 0;
+                real DxiVx, DetVx, DztVx, DxiVy, DetVy, DztVy, DxiVz, DetVz, DztVz;
+                real DxiTxx, DetTxx, DztTxx, DxiTyy, DetTyy, DztTyy, DxiTzz, DetTzz, DztTzz;
+                real DxiTxy, DetTxy, DztTxy, DxiTxz, DetTxz, DztTxz, DxiTyz, DetTyz, DztTyz;
+                if (idz >= o->conindex) {
+                    xiy = o->drv[XIY][G];
+                    xiz = o->drv[XIZ][G];
+                    etx = o->drv[ETX][G];
+                    etz = o->drv[ETZ][G];
+                    ztx = o->drv[ZTX][G];
+                    zty = o->drv[ZTY][G];
 
-                real DxiVx = (xix * DxTxx + xiy * DxTxy + xiz * DxTxz) * rho;
-                real DetVx = (etx * DyTxx + ety * DyTxy + etz * DyTxz) * rho;
-#if PLANT(1) /* zeta_x and zeta_y swapped in the zeta term of the Vx rate */
-                real DztVx = (zty * DzTxx + ztx * DzTxy + ztz * DzTxz) * rho;
-#else
-                real DztVx = (ztx * DzTxx + zty * DzTxy + ztz * DzTxz) * rho;
-#endif
-                real DxiVy = (xix * DxTxy + xiy * DxTyy + xiz * DxTyz) * rho;
-                real DetVy = (etx * DyTxy + ety * DyTyy + etz * DyTyz) * rho;
-                real DztVy = (ztx * DzTxy + zty * DzTyy + ztz * DzTyz) * rho;
-                real DxiVz = (xix * DxTxz + xiy * DxTyz + xiz * DxTzz) * rho;
-                real DetVz = (etx * DyTxz + ety * DyTyz + etz * DyTzz) * rho;
-                real DztVz = (ztx * DzTxz + zty * DzTyz + ztz * DzTzz) * rho;
+                    DxiVx = (xix * DxTxx + xiy * DxTxy + xiz * DxTxz) * rho;
+                    DetVx = (etx * DyTxx + ety * DyTxy + etz * DyTxz) * rho;
+    #if PLANT(1) /* zeta_x and zeta_y swapped in the zeta term of the Vx rate */
+                    DztVx = (zty * DzTxx + ztx * DzTxy + ztz * DzTxz) * rho;
+    #else
+                    DztVx = (ztx * DzTxx + zty * DzTxy + ztz * DzTxz) * rho;
+    #endif
+                    DxiVy = (xix * DxTxy + xiy * DxTyy + xiz * DxTyz) * rho;
+                    DetVy = (etx * DyTxy + ety * DyTyy + etz * DyTyz) * rho;
+                    DztVy = (ztx * DzTxy + zty * DzTyy + ztz * DzTyz) * rho;
+                    DxiVz = (xix * DxTxz + xiy * DxTyz + xiz * DxTzz) * rho;
+                    DetVz = (etx * DyTxz + ety * DyTyz + etz * DyTzz) * rho;
+                    DztVz = (ztx * DzTxz + zty * DzTyz + ztz * DzTzz) * rho;
 
-                real DxiTxx = lam2mu * xix * DxVx + lambda * xiy * DxVy + lambda * xiz * DxVz;
-                real DetTxx = lam2mu * etx * DyVx + lambda * ety * DyVy + lambda * etz * DyVz;
-                real DztTxx = lam2mu * ztx * DzVx + lambda * zty * DzVy + lambda * ztz * DzVz;
-                real DxiTyy = lambda * xix * DxVx + lam2mu * xiy * DxVy + lambda * xiz * DxVz;
-                real DetTyy = lambda * etx * DyVx + lam2mu * ety * DyVy + lambda * etz * DyVz;
-                real DztTyy = lambda * ztx * DzVx + lam2mu * zty * DzVy + lambda * ztz * DzVz;
-                real DxiTzz = lambda * xix * DxVx + lambda * xiy * DxVy + lam2mu * xiz * DxVz;
-                real DetTzz = lambda * etx * DyVx + lambda * ety * DyVy + lam2mu * etz * DyVz;
-                real DztTzz = lambda * ztx * DzVx + lambda * zty * DzVy + lam2mu * ztz * DzVz;
-                real DxiTxy = (xiy * DxVx + xix * DxVy) * miu;
-#if PLANT(2) /* eta_x and eta_y swapped in the eta term of the Txy rate */
-                real DetTxy = (etx * DyVx + ety * DyVy) * miu;
+                    DxiTxx = lam2mu * xix * DxVx + lambda * xiy * DxVy + lambda * xiz * DxVz;
+                    DetTxx = lam2mu * etx * DyVx + lambda * ety * DyVy + lambda * etz * DyVz;
+                    DztTxx = lam2mu * ztx * DzVx + lambda * zty * DzVy + lambda * ztz * DzVz;
+                    DxiTyy = lambda * xix * DxVx + lam2mu * xiy * DxVy + lambda * xiz * DxVz;
+                    DetTyy = lambda * etx * DyVx + lam2mu * ety * DyVy + lambda * etz * DyVz;
+                    DztTyy = lambda * ztx * DzVx + lam2mu * zty * DzVy + lambda * ztz * DzVz;
+                    DxiTzz = lambda * xix * DxVx + lambda * xiy * DxVy + lam2mu * xiz * DxVz;
+                    DetTzz = lambda * etx * DyVx + lambda * ety * DyVy + lam2mu * etz * DyVz;
+                    DztTzz = lambda * ztx * DzVx + lambda * zty * DzVy + lam2mu * ztz * DzVz;
+                    DxiTxy = (xiy * DxVx + xix * DxVy) * miu;
+    #if PLANT(2) /* eta_x and eta_y swapped in the eta term of the Txy rate */
+                    DetTxy = (etx * DyVx + ety * DyVy) * miu;
+    #else
+                    DetTxy = (ety * DyVx + etx * DyVy) * miu;
+    #endif
+                    DztTxy = (zty * DzVx + ztx * DzVy) * miu;
+                    DxiTxz = (xiz * DxVx + xix * DxVz) * miu;
+                    DetTxz = (etz * DyVx + etx * DyVz) * miu;
+                    DztTxz = (ztz * DzVx + ztx * DzVz) * miu;
+                    DxiTyz = (xiz * DxVy + xiy * DxVz) * miu;
+                    DetTyz = (etz * DyVy + ety * DyVz) * miu;
+                    DztTyz = (ztz * DzVy + zty * DzVz) * miu;
+                } else {
+                    DxiVx = rho * xix * DxTxx;
+                    DetVx = rho * ety * DyTxy;
+                    DztVx = ztz * DzTxz * rho;
+                    DxiVy = rho * xix * DxTxy;
+                    DetVy = rho * ety * DyTyy;
+                    DztVy = ztz * DzTyz * rho;
+                    DxiVz = rho * xix * DxTxz;
+                    DetVz = rho * ety * DyTyz;
+                    DztVz = ztz * DzTzz * rho;
+                    DxiTxx = lam2mu * xix * DxVx;
+                    DetTxx = lambda * ety * DyVy;
+                    DztTxx = lambda * ztz * DzVz;
+                    DxiTyy = lambda * xix * DxVx;
+                    DetTyy = lam2mu * ety * DyVy;
+                    DztTyy = lambda * ztz * DzVz;
+                    DxiTzz = lambda * xix * DxVx;
+                    DetTzz = lambda * ety * DyVy;
+#if PLANT(17) /* lambda in place of lam2mu on the zz diagonal of the Cartesian branch (calculate.cu:3940) */
+                    DztTzz = lambda * ztz * DzVz;
 #else
-                real DetTxy = (ety * DyVx + etx * DyVy) * miu;
+                    DztTzz = lam2mu * ztz * DzVz;
 #endif
-                real DztTxy = (zty * DzVx + ztx * DzVy) * miu;
-                real DxiTxz = (xiz * DxVx + xix * DxVz) * miu;
-                real DetTxz = (etz * DyVx + etx * DyVz) * miu;
-                real DztTxz = (ztz * DzVx + ztx * DzVz) * miu;
-                real DxiTyz = (xiz * DxVy + xiy * DxVz) * miu;
-                real DetTyz = (etz * DyVy + ety * DyVz) * miu;
-                real DztTyz = (ztz * DzVy + zty * DzVz) * miu;
+                    DxiTxy = miu * xix * DxVy;
+                    DetTxy = miu * ety * DyVx;
+                    DztTxy = 0.0;
+                    DxiTxz = miu * xix * DxVz;
+                    DetTxz = 0.0;
+                    DztTxz = ztz * DzVx * miu;
+                    DxiTyz = 0.0;
+                    DetTyz = miu * ety * DyVz;
+                    DztTyz = ztz * DzVy * miu;
+                }
 
                 real APDx = o->APD[0][idx], APDy = o->APD[1][idy], APDz = o->APD[2][idz];
                 real Bx = o->Bt[0][idx], By = o->Bt[1][idy], Bz = o->Bt[2][idz];

@@ -54,6 +54,10 @@ int orc_set_moments(orc_t *o, int n, const int *locs, const real *vec, const rea
 int orc_set_focal(orc_t *o, int n, const int *locs, const real *series, int fnt, real fdt);
 int orc_set_receivers(orc_t *o, int n, const int *locs);
 int orc_set_pgv(orc_t *o, int on);
+/* hybrid_grid = 1: the reference's ConIndex (z index with the 3 ghosts);
+ * below it CalWave's Cartesian contraction applies (calculate.cu:3910-3956).
+ * -1 unless 3 <= conindex <= nk: the three surface rows stay curvilinear. */
+int orc_set_conindex(orc_t *o, int conindex);
 
 /* Zero the state and compute the free-surface closure matrices (VelPDcoeff). */
 int orc_init(orc_t *o);

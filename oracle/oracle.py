@@ -60,6 +60,7 @@ def _bind(lib, double):
         "orc_set_focal": (ctypes.c_int, [VP, ctypes.c_int, IP, VP, ctypes.c_int, R]),
         "orc_set_receivers": (ctypes.c_int, [VP, ctypes.c_int, IP]),
         "orc_set_pgv": (ctypes.c_int, [VP, ctypes.c_int]),
+        "orc_set_conindex": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_init": (ctypes.c_int, [VP]),
         "orc_run": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_run_stages": (ctypes.c_int, [VP, ctypes.c_int]),
@@ -170,6 +171,9 @@ class Oracle:
     def set_pgv(self, on=True):
         self.lib.orc_set_pgv(self.h, 1 if on else 0)
 
+    def set_conindex(self, conindex):
+        self._check(self.lib.orc_set_conindex(self.h, int(conindex)), "set_conindex")
+
     def init(self):
         self._check(self.lib.orc_init(self.h), "init")
 
@@ -186,6 +190,8 @@ class Oracle:
         if sc.get("recv_locs") is not None and len(sc["recv_locs"]):
             self.set_receivers(sc["recv_locs"])
         self.set_pgv(sc.get("pgv", True))
+        if sc.get("conindex") is not None:
+            self.set_conindex(sc["conindex"])
         self.init()
 
     def run(self, nsteps):

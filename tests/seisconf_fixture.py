@@ -18,7 +18,8 @@ def surfaces(ni=NI, nj=NJ):
     return X, Y, [top, np.full_like(X, -500.0), np.full_like(X, -900.0), np.full_like(X, -1500.0)]
 
 
-def write_set(d, nt=12, media=True, source="moment", hybrid=0, gp=(5, 4, 6), eq=(1, 0, 1), extra_conf=""):
+def write_set(d, nt=12, media=True, source="moment", hybrid=0, gp=(5, 4, 6), eq=(1, 0, 1), extra_conf="",
+              conv="-3.0E3"):
     d.mkdir(parents=True, exist_ok=True)
     X, Y, surf = surfaces()
     (d / "SeisFD3D.conf").write_text(f"""# main parameters
@@ -32,7 +33,7 @@ seispath = {d}
 grid_type = vmap  # vmap or point
 grid_filename = grid.dat
 hybrid_grid = {hybrid}
-conversion_interface_depth = -3.0E3
+conversion_interface_depth = {conv}
 reference_velocity = 2000.0
 media_type = volume # interface, interface3D or volume
 media_filename = vol.nc

@@ -507,8 +507,48 @@ def pin_outputs_are_step_end_fields(lib=None):
     assert np.array_equal(got[:, 3:-3, 3:-3], pgv[:, 3:-3, 3:-3])
 
 
+M_DIAG = np.diag([1.15, 0.9, 1.3])
+
+
+def pin_hybrid_cartesian_branch(lib=None):
+    """(g) hybrid_grid = 1: below ConIndex CalWave uses a Cartesian form of
+    the contraction (calculate.cu:3910-3956).  On a grid stretched along the
+    three axes (diagonal metrics, as the hybrid mesh builds below the
+    conversion depth) it is exact, so every rate of a linear state, stress
+    and velocity, and every PML auxiliary rate equals the chain-rule closed
+    form; ConIndex puts all rows below the top six in that branch."""
+    sc = affine_scenario(22, 21, 20, 2, nabs=(6, 6, 6, 6, 6, 0), M=M_DIAG)
+    sc["conindex"] = 3 + 20 - 6
+    o = make_oracle(sc, lib)
+    grads, consts = random_linear(13)
+    check_linear_stage(o, sc, grads, consts, FB_TABLE[0][0], M=M_DIAG)
+
+
+def pin_hybrid_equals_curvilinear_on_identity_metrics(lib=None):
+    """On identity metrics the Cartesian and the curvilinear contraction
+    give the same values term by term (products with 1 and sums with zeros
+    are exact), so a hybrid run equals the all-curvilinear run value for
+    value, PML, surface and sources included."""
+    sc = scenario.config_random(24, 22, 18, 8, [4, 4, 3, 3, 4, 0], seed=31, topo_amp=0.0, shear=False)
+    a = make_oracle(sc, lib, double=True)
+    sc2 = dict(sc, conindex=3 + 12)
+    b = make_oracle(sc2, lib, double=True)
+    rng = np.random.default_rng(31)
+    for c, st in enumerate(scenario.random_state(a.shape, rng)):
+        a.set_field(c, st.astype(np.float64))
+        b.set_field(c, st.astype(np.float64))
+    a.run(8)
+    b.run(8)
+    for c in range(9):
+        x, y = a.get_field(c), b.get_field(c)
+        assert np.abs(x).max() > 0
+        assert np.array_equal(x, y), f"{COMPS[c]}: {np.count_nonzero(x != y)} values differ"
+
+
 PINS = {
     "affine_linear_stage": pin_affine_linear_stage,
+    "hybrid_cartesian_branch": pin_hybrid_cartesian_branch,
+    "hybrid_equals_curvilinear_on_identity_metrics": pin_hybrid_equals_curvilinear_on_identity_metrics,
     "fb_schedule": pin_fb_schedule,
     "closure_matrices": pin_closure_matrices,
     "source_magnitudes": pin_source_magnitudes,

@@ -45,6 +45,8 @@ PLANTS = {
          lambda lib: pins_flat.pin_linear_stress_first_moment(False, lib)),
     16: ("RK4: second and fourth stage weights exchanged (calculate.cu:13)",
          pins_flat.pin_rk4_amplification),
+    17: ("hybrid grid: lambda in place of lam2mu on the zz diagonal of the Cartesian branch (calculate.cu:3940)",
+         pins_curv.pin_hybrid_cartesian_branch),
 }
 
 

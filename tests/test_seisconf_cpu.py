@@ -29,6 +29,7 @@ import scenario
 import seisconf as S
 import seisconf_fixture as FX
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hillmodel_cut")
 EXAMPLE = "/root/reference/EXAMPLE/hillmodel"
 F32 = np.float32
 
@@ -81,7 +82,7 @@ def test_vmap_equal_spacing_is_linear_between_anchors(tmp_path):
     conf = FX.write_set(tmp_path, media=False, gp=(5, 4, 6), eq=(1, 1, 1))
     m = S.read_main(str(conf))
     posx, posy, posz, gp, eq = S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK)
-    x, y, z = S.vmap_coords(posx, posy, posz, gp, eq, FX.NK)
+    x, y, z, _ = S.vmap_coords(posx, posy, posz, gp, eq, FX.NK)
     L = S.LFD
     zindex = [L + 15, L + 10, L + 6, L]
     for k in range(4):  # every anchor surface lands exactly on its grid row
@@ -103,7 +104,7 @@ def test_flat_vmap_gives_identity_metrics(tmp_path):
             lines.append(f"{i * dh} {j * dh} 0.0 {-(nk - 1) * dh}")
     (tmp_path / "g.dat").write_text("\n".join(lines) + "\n")
     posx, posy, posz, gp, eq = S.read_vmap(str(tmp_path / "g.dat"), ni, nj, nk)
-    x, y, z = S.vmap_coords(posx, posy, posz, gp, eq, nk)
+    x, y, z, _ = S.vmap_coords(posx, posy, posz, gp, eq, nk)
     met = scenario.metrics_from_coords(x, y, z, dh)
     v = (slice(3, -3),) * 3
     for name, want in (("xix", 1), ("etay", 1), ("zetaz", 1), ("jac", 1), ("xiy", 0), ("xiz", 0),
@@ -146,7 +147,7 @@ def test_vmap_transition_layer(tmp_path):
     conf = FX.write_set(tmp_path, media=False)  # middle layer not equally spaced
     m = S.read_main(str(conf))
     posx, posy, posz, gp, eq = S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK)
-    x, y, z = S.vmap_coords(posx, posy, posz, gp, eq, FX.NK)
+    x, y, z, _ = S.vmap_coords(posx, posy, posz, gp, eq, FX.NK)
     for i, j in ((5, 7), (12, 9), (0, 0)):
         col = z[3 + i, 3 + j, 3:-3].astype(np.float64)
         h = np.diff(col)
@@ -203,7 +204,7 @@ def _reposition_loops(px, py, pz, x, y, z, hyp):
 def test_reposition_matches_loops(tmp_path):
     conf = FX.write_set(tmp_path, media=False)
     m = S.read_main(str(conf))
-    x, y, z = S.vmap_coords(*S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK), FX.NK)
+    x, y, z, _ = S.vmap_coords(*S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK), FX.NK)
     rng = np.random.default_rng(3)
     pts = [(rng.uniform(0, 2300), rng.uniform(0, 1900), rng.uniform(-1500, 100)) for _ in range(6)]
     pts += [(1200.0, 900.0, 9e6), (50.0, 50.0, 9e6),       # surface points
@@ -272,7 +273,7 @@ def test_angle2moment_float():
 def test_volume_media_reproduces_linear_fields(tmp_path):
     conf = FX.write_set(tmp_path)
     m = S.read_main(str(conf))
-    x, y, z = S.vmap_coords(*S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK), FX.NK)
+    x, y, z, _ = S.vmap_coords(*S.read_vmap(m["grid_filename"], FX.NI, FX.NJ, FX.NK), FX.NK)
     alpha, beta, rho = S.volume_media(m["media_filename"], x, y, z)
     v = (slice(3, -3),) * 3
     vp, vs, den = FX.vol_fields(x[v].astype(np.float64), y[v].astype(np.float64), z[v].astype(np.float64))
@@ -304,10 +305,35 @@ def test_load_full_set(tmp_path):
     assert sc["recv_pos"][0, 2] == z[tuple(sc["recv_locs"][0])]
 
 
+def test_hybrid_grid_layers(tmp_path):
+    """hybrid_grid = 1 (gridmesh.cpp:243-431): the layer holding the
+    conversion depth (-1200 m, inside the layer between -1500 and -900 m)
+    ends there, a flat layer runs from it to the shallowest point of the
+    bottom (-1500 m), and the nk planes are shared by thickness (650, 400,
+    300 and 300 m of 1650): int(16*650/1650) = 6, int(16*400/1650) = 3,
+    int(16*300/1650) = 2 and the 4 left for the flat layer, so ConIndex =
+    3 + 4 and the planes below it lie 75 m apart.  The transition layer
+    (equalspacing 0) now spans 3 planes between spacings 650/6 and 150 m."""
+    sc = S.load(str(FX.write_set(tmp_path, hybrid=1, conv="-1200.0")))
+    assert sc["conindex"] == 3 + 4
+    z = sc["coords"][2]
+    for k in range(3, 3 + 5):
+        np.testing.assert_array_equal(z[3:-3, 3:-3, k], F32(-1500.0 + (k - 3) * 75.0))
+    for k in range(3 + 4, 3 + 7):
+        np.testing.assert_array_equal(z[3:-3, 3:-3, k], F32(-1200.0 + (k - 7) * 150.0))
+    np.testing.assert_array_equal(z[3:-3, 3:-3, 3 + 9], F32(-500.0))
+    top = np.vectorize(lambda v: F32(float(f"{v:.6f}")))(FX.surfaces()[2][0])   # as the grid file holds it
+    np.testing.assert_array_equal(z[3:-3, 3:-3, 3 + 15], top)
+    d = np.diff(z[3:-3, 3:-3, 3 + 6:3 + 10].astype(np.float64), axis=2)
+    assert (d > 0).all() and np.isfinite(d).all()
+    # the all-curvilinear grid of the same files has no ConIndex entry
+    assert "conindex" not in S.load(str(FX.write_set(tmp_path / "c", hybrid=0)))
+    # a conversion depth outside every layer
+    with pytest.raises(S.ConfError, match="conversion"):
+        S.load(str(FX.write_set(tmp_path / "o", hybrid=1, conv="-3000.0")))
+
+
 def test_refusals(tmp_path):
-    conf = FX.write_set(tmp_path / "h", hybrid=1)
-    with pytest.raises(S.ConfError, match="hybrid"):
-        S.load(str(conf))
     conf = FX.write_set(tmp_path / "p", media=False)
     conf.write_text(conf.read_text().replace("media_type = volume", "media_type = interface"))
     with pytest.raises(S.ConfError, match="NaN"):
@@ -318,7 +344,7 @@ def test_refusals(tmp_path):
     (tmp_path / "p" / "vol.nc").write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
     conf.write_text(conf.read_text().replace("grid_type = point", "grid_type = vmap")
                     .replace("media_type = interface", "media_type = volume"))
-    with pytest.raises(S.ConfError, match="classic"):
+    with pytest.raises(S.ConfError, match="classic" if S._nc4_library() is None else "does not open"):
         S.load(str(conf))
 
 
@@ -329,17 +355,58 @@ def test_reference_example_files():
     assert m["stept"] == F32(0.003) and m["steph"] == F32(50.0) and m["hybrid_grid"] == 1
     posx, posy, posz, gp, eq = S.read_vmap(m["grid_filename"], 201, 201, 200)
     assert gp == [199] and eq == [True]
-    x, y, z = S.vmap_coords(posx, posy, posz, gp, eq, 200)
+    x, y, z, _ = S.vmap_coords(posx, posy, posz, gp, eq, 200)
     assert (z[3:-3, 3:-3, 3] == F32(-10000.0)).all()
     np.testing.assert_array_equal(z[3:-3, 3:-3, 202], posz[:, :, 0])
     assert S.read_absorb(m["absorb_filename"])[0] == [12, 12, 12, 12, 12, 0]
     st = S.read_stations(m["station_filename"])
     assert len(st["pos"]) == 26 + 9 and st["snapshots"].shape == (1, 11)
     np.testing.assert_array_equal(st["snapshots"][0], [1, 1, 100, 201, 201, 1, 1, 1, 1, 20, 3])
-    with pytest.raises(S.ConfError, match="hybrid"):
-        S.load(f"{EXAMPLE}/SeisFD3D.conf", seispath=EXAMPLE)
-    with pytest.raises(S.ConfError, match="classic"):   # its focal file is NETCDF4
-        S.read_sources(m["source_filename"], EXAMPLE, 10, m["stept"])
+    # its focal file is netCDF-4 (HDF5): read through libcgfd_nc4.so
+    src = S.read_sources(m["source_filename"], EXAMPLE, 10, m["stept"])
+    f = src["focal"]
+    assert f["nt"] == 1500 and f["dt"] == F32(0.006) and f["series"].shape == (6, 1, 1500)
+    np.testing.assert_array_equal(f["pos"], [[5000.0, 5000.0, -5000.0]])
+    # the whole example (hybrid grid, conversion depth -3 km) with stand-in
+    # media, since its media file is not in the reference repository
+    shp = (201 + 6, 201 + 6, 200 + 6)
+    media = (np.full(shp, 6000.0, F32), np.full(shp, 3464.0, F32), np.full(shp, 2700.0, F32))
+    sc = S.load(f"{EXAMPLE}/SeisFD3D.conf", seispath=EXAMPLE, media=media)
+    assert sc["conindex"] > 3 and sc["focal_locs"].shape == (1, 3)
+    z = sc["coords"][2]
+    k = sc["conindex"] - 1
+    assert (z[3:-3, 3:-3, k] == z[3, 3, k]).all()       # flat below the conversion depth
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLE), reason="reference example not present")
+def test_netcdf4_focal_file_equals_its_classic_copy():
+    """The example's netCDF-4 focal file read through libhdf5 equals the
+    classic copy of it in tests/golden/hillmodel_cut (written by
+    make_hillmodel_cut.py), variable by variable."""
+    from scipy.io import netcdf_file
+
+    a = S._read_netcdf(f"{EXAMPLE}/stf-GPU.nc", "focal process file")
+    with netcdf_file(os.path.join(GOLDEN, "stf-classic.nc"), "r", mmap=False) as b:
+        assert a.dimensions["NP"] == b.dimensions["NP"] and a.dimensions["NT"] == b.dimensions["NT"]
+        assert a.DT == F32(b.DT)
+        for nm in ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz", "posx", "posy", "posz"):
+            np.testing.assert_array_equal(a.variables[nm][:], b.variables[nm][:])
+
+
+def test_hillmodel_cut_fixture_loads():
+    """The cut-down copy of the reference example (48 x 48 columns around its
+    focal point, its own anchors, hybrid grid at -3 km) loads: the hybrid
+    layering of its single layer from the Gaussian hill to -10 km puts 124
+    planes below the conversion depth (ConIndex 127), every one flat, and
+    the focal point at 5 km depth under the hill's flank."""
+    sc = S.load(os.path.join(GOLDEN, "SeisFD3D.conf"), seispath=GOLDEN)
+    assert (sc["ni"], sc["nj"], sc["nk"]) == (48, 48, 200) and sc["conindex"] == 127
+    z = sc["coords"][2]
+    for k in range(3, 127):
+        assert (z[3:-3, 3:-3, k] == z[3, 3, k]).all()
+    assert z[3, 3, 126] <= F32(-3000.0) and z[3, 3, 127] == F32(-3000.0)
+    np.testing.assert_array_equal(sc["focal_locs"], [[3 + 24, 3 + 24, 3 + 89]])
+    assert sc["nabs"] == [12, 12, 12, 12, 12, 0] and len(sc["recv_locs"]) == 35
 
 
 def _moduli(v):

@@ -33,6 +33,6 @@ else
     echo "no hdf5.h under $HDF5_ROOT: libcgfd_nc4.so (netCDF-4 inputs) not built"
 fi
 # the C++ host driver (one child-rank run through the C-ABI)
-g++ -O2 -std=c++17 -Wall -I"$root/include" "$here/host/cgfd_child.cpp" -L"$here" -lcgfd3d \
-    -Wl,-rpath,'$ORIGIN' -o "$here/cgfd_child"
+g++ -O2 -std=c++17 -Wall -ffp-contract=off -I"$root/include" "$here/host/cgfd_child.cpp" "$here/host/cgfd_conf.cpp" \
+    -L"$here" -lcgfd3d -ldl -Wl,-rpath,'$ORIGIN' -o "$here/cgfd_child"
 echo "built $here/cgfd_child"

@@ -12,6 +12,7 @@ Arithmetic is FP32 where the reference's `Real` is float, with the reference's
 double-literal promotions, so the tables match what the reference would hand
 to its kernels.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -19,6 +20,23 @@ import numpy as np
 LFD = 3
 F32 = np.float32
 PI_F = F32(3.141592653589793)  # constant::PI is a Real (typenew.h:115)
+
+# float-argument pow/log bind powf/logf in the reference's float build; the
+# C library's own routines keep the tables independent of numpy's SIMD
+# implementations (and equal to a C++ host's)
+_libm = ctypes.CDLL("libm.so.6")
+for _fn in ("powf", "logf"):
+    getattr(_libm, _fn).restype = ctypes.c_float
+_libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+_libm.logf.argtypes = [ctypes.c_float]
+
+
+def _powf(a, b):
+    return F32(_libm.powf(float(a), float(b)))
+
+
+def _logf(a):
+    return F32(_libm.logf(float(a)))
 
 
 # ---------------------------------------------------------------------------
@@ -197,12 +215,12 @@ def cfs_pml_tables(ni, nj, nk, nabs, velabs, bmax, amax, dh, metrics):
                 L[i] = F32(L[i - 1] + F32(steph / grad_norm(ax, i)))
             L0 = L[LFD + n[ax] - 1]
             rng = range(i1, LFD + n[ax])
-        dmax = F32(-float(velabs[side]) / 2.0 / float(L0) * float(np.log(Rpp)) * (float(PD) + 1.0))
+        dmax = F32(-float(velabs[side]) / 2.0 / float(L0) * float(_logf(Rpp)) * (float(PD) + 1.0))
         for i in rng:
             r = F32(L[i] / L0)
-            D[ax][i] = F32(dmax * np.power(r, PD))
-            B[ax][i] = F32(1.0 + (float(bmax[side]) - 1.0) * float(np.power(r, PB)))
-            A[ax][i] = F32(float(amax[side]) * (1.0 - float(np.power(r, PA))))
+            D[ax][i] = F32(dmax * _powf(r, PD))
+            B[ax][i] = F32(1.0 + (float(bmax[side]) - 1.0) * float(_powf(r, PB)))
+            A[ax][i] = F32(float(amax[side]) * (1.0 - float(_powf(r, PA))))
     tabs = {}
     for ax, nm in enumerate("xyz"):
         tabs["APD" + nm] = (A[ax] + D[ax] / B[ax]).astype(F32)

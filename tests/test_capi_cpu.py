@@ -80,7 +80,7 @@ FNO_LIB = os.path.join(ROOT, "copara-cgfd3d_amd", "libcgfd_fno.so")
 @pytest.mark.skipif(not os.path.exists(FNO_LIB), reason="libcgfd_fno.so not built")
 def test_fno_library_exports_and_validates_without_the_gpu():
     names = declared_symbols(FNO_HEADER)
-    assert names == ["cgfd_fno_gemm_bf16"]
+    assert sorted(names) == ["cgfd_fno_adam", "cgfd_fno_gelu_bwd", "cgfd_fno_gemm_bf16", "cgfd_fno_gemm_bf16_ex"]
     lib = ctypes.CDLL(FNO_LIB)
     assert all(hasattr(lib, n) for n in names)
     f = lib.cgfd_fno_gemm_bf16
@@ -91,3 +91,41 @@ def test_fno_library_exports_and_validates_without_the_gpu():
     assert f(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 0, 4, 4, 1, 0, None) == -1
     assert f(None, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 4, 1, 0, None) == -1
     assert f(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 64 * 65536, 4, 4, 1, 0, None) == -1
+
+    # the fused-epilogue product, the GELU backward and Adam reject bad
+    # arguments before any launch
+    g = lib.cgfd_fno_gemm_bf16_ex
+    g.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P, P, L, L, P, I, P]
+    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 4, 1, 0, None, None, 0, 0, None, 7, None) == -1
+    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 0, 1, 0, None, None, 0, 0, None, 1, None) == -1
+    b = lib.cgfd_fno_gelu_bwd
+    b.argtypes = [P, P, P, P, L, P]
+    assert b(None, dummy, dummy, None, 4, None) == -1
+    assert b(dummy, dummy, dummy, None, -1, None) == -1
+    assert b(dummy, dummy, dummy, None, 0, None) == 0          # nothing to do
+    a = lib.cgfd_fno_adam
+    D, LL = ctypes.c_double, ctypes.c_longlong
+    a.argtypes = [P, P, I, D, D, D, D, LL, P]
+    assert a(dummy, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 0, None) == -1     # steps count from 1
+    assert a(dummy, dummy, 1, 1e-3, 1.0, 0.999, 1e-8, 1, None) == -1     # beta1 must be below 1
+    assert a(None, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 1, None) == -1
+    assert a(dummy, dummy, 0, 1e-3, 0.9, 0.999, 1e-8, 1, None) == 0      # no chunks
+
+
+NC4_HEADER = os.path.join(ROOT, "include", "cgfd_nc4.h")
+NC4_LIB = os.path.join(ROOT, "copara-cgfd3d_amd", "libcgfd_nc4.so")
+
+
+@pytest.mark.skipif(not os.path.exists(NC4_LIB), reason="libcgfd_nc4.so not built (no libhdf5)")
+def test_nc4_library_exports_every_declared_symbol(tmp_path):
+    names = declared_symbols(NC4_HEADER)
+    assert sorted(names) == ["cgfd_nc4_get_att_float", "cgfd_nc4_get_var_double", "cgfd_nc4_get_var_float",
+                             "cgfd_nc4_probe", "cgfd_nc4_var_dims"]
+    lib = ctypes.CDLL(NC4_LIB)
+    assert all(hasattr(lib, n) for n in names)
+    lib.cgfd_nc4_probe.argtypes = [ctypes.c_char_p]
+    bad = tmp_path / "x.nc"
+    bad.write_bytes(b"CDF\x01" + b"\0" * 60)
+    assert lib.cgfd_nc4_probe(None) == -1
+    assert lib.cgfd_nc4_probe(str(bad).encode()) == 0              # classic, not HDF5
+    assert lib.cgfd_nc4_probe(str(tmp_path / "missing.nc").encode()) == 0

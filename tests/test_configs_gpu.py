@@ -10,10 +10,13 @@
 - A fused batch of 4 uneven hill scenarios drawn from the 96-192 x 96-192 x
   64-128 distribution: 150 steps in one cgfd_run_batch call equal each
   scenario run alone, and the smallest of them equals the oracle.
-- The 192x192x128 Gaussian-hill medium of the surrogate: FD data from fused
-  batches equal single runs, and the trained network's forward pass and
-  gate run on its 128x128 surface maps (reduced counts: the full 4096-run
-  training set is a benchmark, not a test).
+- The 192x192x128 Gaussian-hill medium of the surrogate, 1024 steps: FD
+  data (128x128 PGV window and 32 stations' traces) from a fused batch,
+  one scenario of which equals its run alone; the 4-layer, width-64,
+  16-mode network with its seismogram head trained for a few epochs; its
+  forward pass and the gate, whose fallback answers are the FD answers
+  (reduced counts: the full 4096-run training set is a benchmark, not a
+  test).
 
 The 64^3 flat configuration runs in full (600 steps) in test_parity_gpu.py.
 Value equality is the criterion throughout (stricter than relative L2 1e-5).
@@ -128,3 +131,33 @@ def test_uneven_batch_full_size_scenarios():
     o.close()
     for g in sims:
         g.close()
+
+
+def test_surrogate_192x192x128_reduced_counts():
+    import surrogate
+
+    base = surrogate.base_medium(ni=192, nj=192, nk=128, nt=1024, dt=0.006, npml=10, window=128, stations=32,
+                                 depth=(20, 101))
+    ptr = surrogate.draw_sources(12, base, seed=11)
+    pte = surrogate.draw_sources(4, base, seed=12)
+    print("FD data, 16 scenarios of 192x192x128 x 1024 steps", flush=True)
+    y, tr = surrogate.run_fd(base, np.concatenate([ptr, pte]), group=16, traces=True)
+    assert y.shape == (16, 3, 128, 128) and tr.shape == (16, 32, 3, 1024)
+    assert np.isfinite(y).all() and (y > 0).all() and np.abs(tr).max() > 0
+    # the fused batch's answer for one scenario is its run alone
+    y1, tr1 = surrogate.run_fd(base, pte[:1], group=1, traces=True, graph=False)
+    np.testing.assert_array_equal(y1[0], y[12])
+    np.testing.assert_array_equal(tr1[0], tr[12])
+    s = surrogate.Surrogate(base, width=64, modes=16, layers=4)
+    losses = s.fit(ptr, y[:12], epochs=4, batch=8, lr=3e-3, augment=True, traces=tr[:12])
+    print("losses", [round(v, 4) for v in losses], flush=True)
+    assert all(np.isfinite(losses))
+    pred, ind, ptr_tr = s.predict(pte, traces=True)
+    assert pred.shape == (4, 3, 128, 128) and ptr_tr.shape == (4, 32, 3, 1024) and ind.shape == (4,)
+    assert np.isfinite(pred).all() and np.isfinite(ptr_tr).all() and (ind >= 0).all()
+    s.calibrate(pte, y[12:], target_err=0.05)
+    s.threshold = float(np.median(ind))
+    got, ok = s.answer(pte, fd_group=4)
+    assert ok.any() and not ok.all()
+    np.testing.assert_array_equal(got[~ok], y[12:][~ok])
+    np.testing.assert_array_equal(got[ok], pred[ok])

@@ -81,3 +81,41 @@ def test_driver_matches_ctypes_run(tmp_path, double):
     with netcdf_file(tmp_path / "seismo.nc", "r", mmap=False) as f:
         assert_same(f.variables["Txy"].data, seis[6].T, "seismo Txy")
     assert np.abs(seis).max() > 0
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_reference_example_from_its_parameter_files(tmp_path):
+    """cgfd_child --conf: the cut-down reference example (hybrid grid, focal
+    source, its receivers and lines) prepared by the C++ front end and run
+    from compiled code gives the traces, PGV and seismo.nc of the same run
+    prepared by seisconf.load and driven through ctypes."""
+    cgfd3d = pytest.importorskip("cgfd3d")
+    import seisconf
+
+    if not os.path.exists(DRIVER):
+        pytest.fail("cgfd_child is not built (copara-cgfd3d_amd/build.sh)")
+    golden = os.path.join(ROOT, "tests", "golden", "hillmodel_cut")
+    steps = 30
+    print("driver run from SeisFD3D.conf", flush=True)
+    r = subprocess.run([DRIVER, "--conf", os.path.join(golden, "SeisFD3D.conf"), "--seispath", golden,
+                        "--out", str(tmp_path), "--steps", str(steps)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sc = seisconf.load(os.path.join(golden, "SeisFD3D.conf"), seispath=golden)
+    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    g.setup(sc)
+    g.run(steps)
+    seis = g.get_seismo()
+    traces = np.fromfile(tmp_path / "traces.bin", np.float32).reshape(seis.shape)
+    assert_same(traces, seis, "traces")
+    pv = np.fromfile(tmp_path / "pgv.bin", np.float32).reshape(3, sc["ni"] + 6, sc["nj"] + 6)
+    assert_same(pv[:, 3:-3, 3:-3], g.get_pgv()[:, 3:-3, 3:-3], "PGV")
+    assert np.abs(seis[:3, :, :steps]).max() > 0
+    ref = tmp_path / "py"
+    ref.mkdir()
+    pos = sc["recv_pos"]
+    g.write_seismo(str(ref / "seismo.nc"), sc["seismo_tinv"], sc["recv_pnum"], sc["recv_lnum"], pos[:, 0], pos[:, 1],
+                   pos[:, 2])
+    with netcdf_file(tmp_path / "seismo.nc", "r", mmap=False) as f, netcdf_file(ref / "seismo.nc", "r", mmap=False) as h:
+        for name in h.variables:
+            assert_same(f.variables[name].data, h.variables[name].data, f"seismo.nc {name}")
+    g.close()

@@ -235,3 +235,117 @@ def test_symmetry_averaged_prediction_is_equivariant():
         sg, want = surrogate.transform(base, src, pred, g)
         got, _ = s.predict(sg, rows=True)
         assert np.abs(got / want - 1).max() < 1e-4, g
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,N,K,batch,off,accumulate", [
+    (64, 64, 32, 1, 0, False), (70, 33, 45, 3, 0, True), (130, 96, 200, 2, 0, False),
+    (1000, 48, 7, 1, 0, True), (72, 80, 40, 2, 3, False)])
+def test_gemm_epilogue_matches_numpy(M, N, K, batch, off, accumulate):
+    """pre = (C +) A B + bias + D and C = GELU(pre) against float64 numpy on
+    the same bf16 operands (GELU from the error function, as torch's)."""
+    from math import erf, sqrt
+
+    rng = np.random.default_rng(M * N + K)
+    a = rng.standard_normal((batch, M, K)).astype(np.float32)
+    b = rng.standard_normal((batch, K, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    d = rng.standard_normal((batch, M, N)).astype(np.float32)
+    c0 = rng.standard_normal((batch, M, N)).astype(np.float32)
+
+    def dev(t):
+        if not off:
+            return t.cuda()
+        buf = torch.zeros(t.numel() + off, dtype=t.dtype, device="cuda")
+        buf[off:] = t.reshape(-1).cuda()
+        return buf[off:].view(t.shape)
+
+    A, B = dev(_bf16(a)), dev(_bf16(b))
+    out = torch.from_numpy(c0).cuda()
+    pre = torch.empty_like(out)
+    surrogate.gemm(A, B, out, accumulate=accumulate, bias=torch.from_numpy(bias).cuda(),
+                   addend=torch.from_numpy(d).cuda(), pre=pre, act=surrogate.ACT_GELU)
+    ar, br = A.float().cpu().numpy().astype(np.float64), B.float().cpu().numpy().astype(np.float64)
+    want_pre = ar @ br + bias + d + (c0 if accumulate else 0.0)
+    gelu = np.vectorize(lambda v: 0.5 * v * (1.0 + erf(v / sqrt(2.0))))
+    scale = np.linalg.norm(want_pre)
+    assert np.linalg.norm(pre.cpu().numpy() - want_pre) <= 1e-5 * scale
+    assert np.linalg.norm(out.cpu().numpy() - gelu(want_pre)) <= 1e-5 * scale
+
+
+@pytest.mark.gpu
+def test_gelu_backward_matches_torch():
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.standard_normal(100003).astype(np.float32) * 3).cuda()
+    gy = torch.from_numpy(rng.standard_normal(100003).astype(np.float32)).cuda()
+    xr = x.clone().requires_grad_(True)
+    torch.nn.functional.gelu(xr).backward(gy)
+    gx, gxb = surrogate.gelu_backward(gy, x)
+    want = xr.grad
+    assert float((gx - want).abs().max()) <= 2e-6 * float(want.abs().max())
+    assert torch.equal(gxb, gx.to(torch.bfloat16))     # round to nearest even, as torch
+    g2, _ = surrogate.gelu_backward(gy, x, gx=gy.clone(), want_bf16=False)
+    assert torch.equal(g2, gx)
+
+
+@pytest.mark.gpu
+def test_adam_matches_torch_adam():
+    """Five steps with a changing learning rate over tensors of several sizes
+    (one spanning many 8192-element chunks): the project's kernel against
+    torch.optim.Adam on the same gradients."""
+    rng = np.random.default_rng(4)
+    shapes = [(64,), (13, 7), (3, 100, 50), (40000,)]
+    ours = [torch.nn.Parameter(torch.from_numpy(rng.standard_normal(s).astype(np.float32)).cuda()) for s in shapes]
+    ref = [torch.nn.Parameter(p.detach().clone()) for p in ours]
+    opt = surrogate.Adam(ours, lr=3e-3)
+    topt = torch.optim.Adam(ref, lr=3e-3)
+    for step in range(5):
+        lr = surrogate.cosine_lr(3e-3, step, 5)
+        grads = [torch.from_numpy(rng.standard_normal(s).astype(np.float32)).cuda() for s in shapes]
+        opt.zero_grad()
+        for p, g in zip(ours, grads):
+            p.grad += g
+        for p, g in zip(ref, grads):
+            p.grad = g.clone()
+        for grp in topt.param_groups:
+            grp["lr"] = lr
+        opt.step(lr)
+        topt.step()
+    for p, q in zip(ours, ref):
+        assert float((p.detach() - q.detach()).abs().max()) <= 1e-6 * float(q.detach().abs().max()) + 1e-9
+
+
+def test_station_layout_is_closed_under_the_symmetries():
+    base = surrogate.base_medium(ni=192, nj=192, nk=24, nt=8, window=128, stations=32)
+    st = {tuple(p) for p in base["stations"]}
+    assert len(st) == 32 and base["window"] == (32, 32, 128)
+    assert all(38 <= i < 154 and 38 <= j < 154 for i, j in st)
+    for i, j in st:
+        assert (j, i) in st and (191 - i, 191 - j) in st
+    tr = np.random.default_rng(0).standard_normal((2, 32, 3, 5)).astype(np.float32)
+    for g in surrogate.SYM:
+        t2 = surrogate.transform_traces(base, tr, g)
+        np.testing.assert_array_equal(surrogate.transform_traces(base, t2, g), tr)
+    # the swap moves station (i, j)'s Vx to station (j, i)'s Vy
+    t2 = surrogate.transform_traces(base, tr, (1, 0, 0))
+    k = 0
+    i, j = base["stations"][k]
+    kk = [tuple(p) for p in base["stations"]].index((j, i))
+    np.testing.assert_array_equal(t2[:, kk, 1], tr[:, k, 0])
+
+
+@pytest.mark.gpu
+def test_fd_traces_are_symmetric_under_the_transforms():
+    # station traces of a transformed source equal the transformed traces:
+    # the swap exactly, the mirror of both axes to 1 %
+    base = surrogate.base_medium(ni=48, nj=48, nk=32, nt=200, dt=0.006, npml=8, window=40, stations=8)
+    src = surrogate.source_rows(surrogate.draw_sources(1, base, seed=6))
+    gs = [g for g in surrogate.SYM if g != (0, 0, 0)]
+    pairs = [surrogate.transform(base, src, None, g)[0] for g in gs]
+    ys, trs = surrogate.run_fd(base, np.concatenate([src] + pairs), group=4, rows=True, traces=True)
+    assert trs.shape == (4, 8, 3, 200) and np.abs(trs).max() > 0
+    for k, g in enumerate(gs):
+        want = surrogate.transform_traces(base, trs[:1], g)
+        err = float(surrogate.trace_error(trs[k + 1:k + 2], want)[0])
+        print(f"trace symmetry {g}: {err:.4f}", flush=True)
+        assert err < (1e-6 if g == (1, 0, 0) else 1e-2), (g, err)

@@ -251,11 +251,15 @@ def surrogate_study(medium, ntrain, ncal, ntest, epochs, width, modes, batch, au
         thr = s.calibrate(pcal, ycal, target_err=0.05)
         out, t_sur = surrogate.timed(lambda: s.predict(pte, traces=traces))
         pred, ind = out[:2]
-        (ans, ok), t_gate = surrogate.timed(lambda: s.answer(pte))
+        if tta:  # the reported configuration: answered end to end, FD for the rejected
+            (ans, ok), t_gate = surrogate.timed(lambda: s.answer(pte))
+        else:
+            ans, ok, t_gate = None, ind <= thr, None
         err = surrogate.pgv_error(pred, yte)
         terr = surrogate.trace_error(out[2], tte) if traces else None
         res[tta] = {"threshold": thr, "t_sur": t_sur, "t_gate": t_gate, "ok": ok, "err": err, "terr": terr,
-                    "err_pw": surrogate.pgv_pointwise_error(pred, yte), "err_gate": surrogate.pgv_error(ans, yte),
+                    "err_pw": surrogate.pgv_pointwise_error(pred, yte),
+                    "err_gate": None if ans is None else surrogate.pgv_error(ans, yte),
                     "rho": float(np.corrcoef(rank(ind), rank(err))[0, 1]) if len(err) > 1 else None}
         log(f"tta={tta}: median PGV error {np.median(err):.4f}, accepted {ok.mean():.3f}")
     r = res[True]

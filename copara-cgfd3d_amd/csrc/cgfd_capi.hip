@@ -1267,6 +1267,8 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
     }
     const int *starts_d = (const int *)(img_d + sofs);
     const int hmet = c0->hmet && c0->use_hmet ? 1 : 0;
+    int hybrid = 0; // one grid with Cartesian planes selects the kernel that has the branch
+    for (int i = 0; i < n; i++) hybrid |= ctxs[i]->zcon > 0 ? 1 : 0;
     for (int t = 0; t < nsteps; t++) {
         for (int k = 0; k < 4; k++) {
             int sx, sz;
@@ -1274,8 +1276,8 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
             launch_timg<R>(sx, sz, ni, nj, cols, TimgArgs<R>{},
                            (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n, 4, c0->stream);
-            launch_stage_batch<R>(sx, sz, k, hmet, (const StageArgs<R> *)(img_d + k * kStageArgsBytes), starts_d, n,
-                                  starts[n], tinc, c0->stream);
+            launch_stage_batch<R>(sx, sz, k, hmet, hybrid, (const StageArgs<R> *)(img_d + k * kStageArgsBytes),
+                                  starts_d, n, starts[n], tinc, c0->stream);
         }
         launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
         for (int i = 0; i < n; i++) {

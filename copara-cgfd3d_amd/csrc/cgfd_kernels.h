@@ -95,8 +95,8 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 // rows of a stage-kernel tile (64 columns wide) in this build
 int stage_tile_rows(bool fp64);
 template <typename R>
-void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
-                        int nblocks, int tinc, hipStream_t s);
+void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, const StageArgs<R> *pa, const int *starts,
+                        int nctx, int nblocks, int tinc, hipStream_t s);
 // traction image: of one grid with its arguments av (pa null, nctx 1), or of
 // nctx grids whose blocks lie at kCtxArgBytes strides from pa (av unused);
 // ni x nj covers every grid's surface, cols is their total column count

@@ -255,7 +255,7 @@ __device__ __forceinline__ void store_aux(AP a, int ax, long aidx, const AuxIn<R
 // rates with their RK update, the traction-image override and the point
 // sources.  The RK update of the wavefield is rk_store (below).
 //   w: z window of the stage input (slot CI is the cell itself)
-template <typename R, int SX, int SZ, int ST, typename AP>
+template <typename R, int SX, int SZ, int ST, int HY, typename AP>
 __device__ __forceinline__ void cell_rates(AP a, const Column<R> &k, const CellIn<R> &in, R (&Dx)[NF], R (&Dy)[NF],
                                            R (&Dz)[NF], const R (&w)[NF][5], int x, int y, int z, int &e,
                                            int tindex, R (&hv)[NF])
@@ -304,11 +304,12 @@ __device__ __forceinline__ void cell_rates(AP a, const Column<R> &k, const CellI
     // planes below the conversion depth (z < zcon, uniform over the
     // workgroup) take CalWave's Cartesian form, which uses only xi_x, eta_y
     // and zeta_z in its own operand order (calculate.cu:3910-3956);
-    // elsewhere the curvilinear contraction of CalWaveCL (4855-4930).
+    // elsewhere the curvilinear contraction of CalWaveCL (4855-4930).  HY = 0
+    // compiles the curvilinear form alone (grids without a Cartesian part).
     R DxiVx, DetVx, DztVx, DxiVy, DetVy, DztVy, DxiVz, DetVz, DztVz;
     R DxiTxx, DetTxx, DztTxx, DxiTyy, DetTyy, DztTyy, DxiTzz, DetTzz, DztTzz;
     R DxiTxy, DetTxy, DztTxy, DxiTxz, DetTxz, DztTxz, DxiTyz, DetTyz, DztTyz;
-    if (z >= a->zcon) {
+    if (!HY || z >= a->zcon) {
         DxiVx = (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ]) * rho;
         DetVx = (etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ]) * rho;
         DztVx = (ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]) * rho;
@@ -591,7 +592,7 @@ __device__ __forceinline__ void finish_cell(AP a, const Column<R> &k, const Cell
                                             int &e, long fs, int tindex)
 {
     R hv[NF];
-    cell_rates<R, SX, SZ, ST>(a, k, in, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
+    cell_rates<R, SX, SZ, ST, 1>(a, k, in, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
     rk_store<R, SZ, ST>(a, in, w, hv, pb, fs);
     pgv_update<R, ST>(a, k, in, hv, z - (geom_of(a).nk - 3));
 }
@@ -866,7 +867,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         // flight while the previous plane's prefetch is consumed.
         const unsigned pbz = pbase(z) + rel_c;
         CellIn<R> cin;
-        load_cell<R, ST, HM>(al, pbz, fs, cin);
+        load_cell<R, ST, (HM & 1)>(al, pbz, fs, cin);
         __syncthreads(); // the previous plane's LDS reads are done (its tile is slot(z+HD) now)
         const int sc = slot(z), sh = slot(z + HD);
 #pragma unroll
@@ -907,7 +908,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
                 }
                 Dz[c] = dfd<R, SZ>(w[c][0], w[c][1], w[c][2], w[c][3], w[c][4], h, rh);
             }
-            cell_rates<R, SX, SZ, ST>(al, k, cin, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
+            cell_rates<R, SX, SZ, ST, (HM >> 1)>(al, k, cin, Dx, Dy, Dz, w, x, y, z, e, tindex, hv);
         }
         // lanes outside the grid: their buffer stores are dropped (FP32), or
         // their zero inputs and zero rates store zeros (FP64)
@@ -1674,10 +1675,14 @@ static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs
         constexpr int ty = lds_ty<R>();
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
         const dim3 grid((unsigned)(nblocks > 0 ? nblocks : ntx * nty * zchunks));
-        if (a.hmet)
-            stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc);
-        else
-            stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc);
+        // mode bit 0: constant horizontal metrics; bit 1: a hybrid grid's
+        // Cartesian planes
+        switch ((a.hmet ? 1 : 0) | (a.zcon > 0 ? 2 : 0)) {
+        case 0: stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
+        case 1: stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
+        case 2: stage_lds_kernel<R, SX, SZ, ST, 2><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
+        default: stage_lds_kernel<R, SX, SZ, ST, 3><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
+        }
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
         stage_split_kernel<R, SX, SZ, ST, 0><<<dim3(ntx * nty * zchunks), dim3(64, SPLIT_TY), 0, s>>>(pa, tinc);
@@ -1711,12 +1716,13 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 }
 
 template <typename R>
-void launch_stage_batch(int sx, int sz, int st, int hmet, const StageArgs<R> *pa, const int *starts, int nctx,
-                        int nblocks, int tinc, hipStream_t s)
+void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, const StageArgs<R> *pa, const int *starts,
+                        int nctx, int nblocks, int tinc, hipStream_t s)
 {
     if (nctx < 1 || nblocks < 1) return;
-    StageArgs<R> a{}; // only the metric mode steers a batched launch
+    StageArgs<R> a{}; // only the metric mode and the hybrid flag steer a batched launch
     a.hmet = hmet;
+    a.zcon = hybrid ? 1 : 0;
     if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
     else if (sx > 0) launch_stage_sxz<R, 1, -1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
     else if (sz > 0) launch_stage_sxz<R, -1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
@@ -1850,7 +1856,7 @@ void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s)
 #define CGFD_INSTANTIATE(R)                                                                                   \
     template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int, \
                                    hipStream_t);                                                              \
-    template void launch_stage_batch<R>(int, int, int, int, const StageArgs<R> *, const int *, int, int, int,   \
+    template void launch_stage_batch<R>(int, int, int, int, int, const StageArgs<R> *, const int *, int, int, int, \
                                         hipStream_t);                                                         \
     template void launch_timg<R>(int, int, int, int, long, const TimgArgs<R> &, const TimgArgs<R> *, int, int, \
                                   hipStream_t);                                                               \

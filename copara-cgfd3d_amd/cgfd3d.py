@@ -50,6 +50,7 @@ SIGNATURES = {
     "cgfd_get_snap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
     "cgfd_set_pgv": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_set_hybrid": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_set_simplecar": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_init": (ctypes.c_int, [_VP]),
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -310,11 +311,18 @@ class Simulation:
         planes); below it the Cartesian contraction applies."""
         self._check(self.lib.cgfd_set_hybrid(self.h, int(conindex)), "set_hybrid")
 
+    def set_simplecar(self, on=True):
+        """The reference's SimpleCar build: xi/eta metrics taken as the
+        identity (call before set_metrics)."""
+        self._check(self.lib.cgfd_set_simplecar(self.h, int(bool(on))), "set_simplecar")
+
     def init(self):
         self._check(self.lib.cgfd_init(self.h), "init")
 
     def setup(self, sc):
         """Load every input of a scenario dict (see scenario.py) and init."""
+        if sc.get("simplecar"):
+            self.set_simplecar(True)
         self.set_metrics(sc["metrics"])
         self.set_media(sc["alpha"], sc["beta"], sc["rho"])
         self.set_pml(sc["nabs"], sc["pml"])

@@ -59,6 +59,7 @@ struct cgfd_ctx {
     bool hmet = false;     // xi/eta metrics are the constants 1/0 in every valid cell
     bool use_hmet = true;  // let the stage kernel exploit it
     int zcon = 0;          // hybrid grid: engine planes z < zcon use the Cartesian contraction
+    bool simplecar = false; // the reference's SimpleCar build: xi/eta metrics taken as the identity
 
     // point sources: host copies until init builds the device tables
     int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
@@ -409,12 +410,31 @@ template <typename U> static bool horizontal_metrics_const(const cgfd_ctx *c, co
 
 extern "C" {
 
-int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv)
+int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv_in)
 {
-    if (!c || !drv) return fail(c, CGFD_EINVAL, "null argument");
+    if (!c || !drv_in) return fail(c, CGFD_EINVAL, "null argument");
     HIPCHK(c, hipSetDevice(c->dev));
+    for (int k = 0; k < NDRV; k++)
+        if (!drv_in[k]) return fail(c, CGFD_EINVAL, "null metric array");
+    // SimpleCar: the xi/eta metrics are the constants 1 (xi_x, eta_y) and 0,
+    // as the reference's SimpleCar build assumes them (calculate.cu:3369-3375,
+    // 4339-4456, 5729-5740); with them the curvilinear and Cartesian forms
+    // compute that build's values (a product with 1 and a sum with a zero are
+    // exact)
+    const void *drv[NDRV];
+    std::vector<unsigned char> one, zero;
+    for (int k = 0; k < NDRV; k++) drv[k] = drv_in[k];
+    if (c->simplecar) {
+        const size_t n = (size_t)(c->g.ni + 6) * (c->g.nj + 6) * (c->g.nk + 6);
+        one.assign(n * c->esz, 0);
+        zero.assign(n * c->esz, 0);
+        for (size_t i = 0; i < n; i++) {
+            if (c->prec == CGFD_FP32) ((float *)one.data())[i] = 1.0f;
+            else ((double *)one.data())[i] = 1.0;
+        }
+        for (int k = XIX; k <= ETZ; k++) drv[k] = (k == XIX || k == ETY) ? one.data() : zero.data();
+    }
     for (int k = 0; k < NDRV; k++) {
-        if (!drv[k]) return fail(c, CGFD_EINVAL, "null metric array");
         int rc = upload_field(c, drv[k], c->met + (size_t)k * c->g.fs * c->esz);
         if (rc) return rc;
     }
@@ -628,6 +648,16 @@ int cgfd_snap_shape(const cgfd_ctx *c, int isnap, int *npoints, int *ntime)
     if (!c || isnap < 0 || isnap >= (int)c->snaps.size()) return CGFD_EINVAL;
     if (npoints) *npoints = c->snaps[isnap].npts;
     if (ntime) *ntime = c->snaps[isnap].ntime;
+    return 0;
+}
+
+int cgfd_set_simplecar(cgfd_ctx *c, int on)
+{
+    if (!c) return CGFD_EINVAL;
+    if (c->have_metrics && c->simplecar != (on != 0))
+        return fail(c, CGFD_EINVAL, "cgfd_set_simplecar must precede cgfd_set_metrics (it replaces the xi/eta metrics "
+                                    "as they are uploaded)");
+    c->simplecar = on != 0;
     return 0;
 }
 

@@ -7,8 +7,9 @@
 // 1. From the reference's own parameter files, as its master prepares them
 //    (cgfd_conf.cpp: SeisFD3D.conf and the grid, absorb, source, station
 //    and media files it names):
-//      cgfd_child --conf <SeisFD3D.conf> [--seispath <dir>] --out <dir> [--steps N]
-//    runs the simulation and writes the outputs into <dir>;
+//      cgfd_child --conf <SeisFD3D.conf> [--seispath <dir>] [--simplecar] --out <dir> [--steps N]
+//    runs the simulation and writes the outputs into <dir> (--simplecar: as
+//    the reference's SimpleCar build computes it);
 //      cgfd_child --conf <SeisFD3D.conf> [--seispath <dir>] --dump <dir>
 //    writes the prepared arrays in the raw format below (and coords.bin,
 //    recv_pos.bin / recv_pnum.bin / recv_lnum.bin) without touching a GPU.
@@ -78,7 +79,7 @@ void write_bin(const std::string &dir, const char *name, const std::vector<T> &v
 template <typename Real>
 struct Inputs {
     cgfd_dims d{};
-    int pgv = 1, conindex = 0, focal_nt = 0, seismo_tinv = 1;
+    int pgv = 1, conindex = 0, focal_nt = 0, seismo_tinv = 1, simplecar = 0;
     double focal_dt = 0;
     std::vector<Real> metrics, media, pml, force_vec, force_stf, moment_vec, moment_stf, focal_series;
     std::vector<int> nabs, force_loc, moment_loc, focal_loc, recv, snap;
@@ -97,6 +98,7 @@ int run(const Inputs<Real> &in, int steps, const std::string &out) {
     cgfd_ctx *c = nullptr;
     check(c, cgfd_create(&d, sizeof(Real) == 8 ? CGFD_FP64 : CGFD_FP32, 0, &c), "create");
 
+    if (in.simplecar) check(c, cgfd_set_simplecar(c, 1), "simplecar");   // the SimpleCar build
     const void *drv[10];
     for (int i = 0; i < 10; ++i) drv[i] = in.metrics.data() + i * N;
     check(c, cgfd_set_metrics(c, drv), "metrics");                  // ParH2D derivF
@@ -249,12 +251,16 @@ void dump(const cgfd_conf::Scenario &s, const std::string &dir) {
 int main(int argc, char **argv) {
     if (argc < 2)
         die("usage: cgfd_child <dir> [steps] | cgfd_child --conf <SeisFD3D.conf> [--seispath <dir>] "
-            "(--dump <dir> | --out <dir> [--steps N])");
+            "[--simplecar] (--dump <dir> | --out <dir> [--steps N])");
     if (std::strcmp(argv[1], "--conf") == 0) {
         std::string conf, seispath, dumpdir, outdir;
-        int steps = -1;
+        int steps = -1, simplecar = 0;
         for (int a = 1; a < argc; ++a) {
             const std::string k = argv[a];
+            if (k == "--simplecar") {
+                simplecar = 1;
+                continue;
+            }
             if (a + 1 >= argc) die("missing value after " + k);
             const std::string v = argv[++a];
             if (k == "--conf") conf = v;
@@ -275,7 +281,9 @@ int main(int argc, char **argv) {
             dump(s, dumpdir);
             return 0;
         }
-        return run<float>(from_conf(s), steps < 0 ? s.nt : steps, outdir);
+        Inputs<float> in = from_conf(s);
+        in.simplecar = simplecar;
+        return run<float>(in, steps < 0 ? s.nt : steps, outdir);
     }
     const std::string dir = argv[1];
     FILE *f = std::fopen((dir + "/dims.txt").c_str(), "r");

@@ -1221,11 +1221,13 @@ def interface3d_media(path, nx, ny, x0, y0, dx, dy, nint, x, y, z, smooth=None):
 # ---------------------------------------------------------------------------
 # the whole run
 # ---------------------------------------------------------------------------
-def load(conf_path, seispath=None, media=None, zwmedia=True):
+def load(conf_path, seispath=None, media=None, zwmedia=True, simplecar=False):
     """Scenario dict for `cgfd3d.Simulation.setup` from a SeisFD3D.conf.
 
     `seispath` overrides the directory the conf file names; `media` =
-    (alpha, beta, rho) arrays with ghosts replace the configured media.
+    (alpha, beta, rho) arrays with ghosts replace the configured media;
+    `simplecar` asks for the reference's SimpleCar build (a compile-time
+    option there: xi/eta metrics taken as the identity).
     The dict also holds the receiver metadata (`recv_pos`, `recv_pnum`,
     `recv_lnum`, `seismo_tinv`) that `Simulation.write_seismo` takes."""
     m = read_main(conf_path, seispath)
@@ -1265,6 +1267,8 @@ def load(conf_path, seispath=None, media=None, zwmedia=True):
               coords=(x, y, z), name=f"conf-{os.path.basename(os.path.dirname(os.path.abspath(conf_path)))}")
     if m["hybrid_grid"]:
         sc["conindex"] = conindex
+    if simplecar:
+        sc["simplecar"] = True
     sc.update(locate_sources(src, x, y, z, rho, beta))
     st = read_stations(m["station_filename"])
     sc["recv_locs"] = locate_stations(st, x, y, z)

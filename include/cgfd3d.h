@@ -117,6 +117,14 @@ int cgfd_set_pgv(cgfd_ctx *ctx, int on);
  * 3 (= nk1), is the all-curvilinear grid of hybrid_grid = 0.  CGFD_EINVAL
  * unless 3 <= conindex <= nk (the three surface rows stay curvilinear). */
 int cgfd_set_hybrid(cgfd_ctx *ctx, int conindex);
+/* The reference's SimpleCar build (its Makefile default), selected there at
+ * compile time: CalDiffmix/CalWavemix/CalTIMGmix in place of
+ * CalDiff/CalWave/CalTIMG and VelPDcoeff with e11 = e22 = 1 and the other
+ * xi/eta entries 0 (calculate.cu:1471-1480, 1509-1511, 3362-3375): the
+ * xi/eta metrics are taken as the identity whatever the metric arrays hold.
+ * Call before cgfd_set_metrics; on grids whose xi/eta metrics are the
+ * identity (uniform x/y spacing) both builds give the same values. */
+int cgfd_set_simplecar(cgfd_ctx *ctx, int on);
 
 /* VelCoeff (calculate.cu:1336-1353) plus state reset: zero the wavefield,
  * auxiliary fields, traces and PGV; compute the traction-free closure

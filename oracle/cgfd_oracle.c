@@ -83,6 +83,7 @@ struct orc {
     int pgv_on;
     real *pgv[3];
     int conindex; /* ConIndex: the curvilinear contraction applies at idz >= conindex */
+    int simplecar; /* the SimpleCar build: xi/eta metrics taken as the identity */
 };
 
 static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
@@ -316,6 +317,17 @@ int orc_set_conindex(orc_t *o, int conindex)
     return 0;
 }
 
+/* The reference's SimpleCar build (its Makefile default): the xi/eta metric
+ * arrays are never allocated and CalDiffmix/CalWavemix/CalTIMGmix replace
+ * CalDiff/CalWave/CalTIMG (calculate.cu:1471-1480, 1509-1511), while
+ * VelPDcoeff takes e11 = e22 = 1 and the other xi/eta entries 0
+ * (calculate.cu:3362-3375). */
+int orc_set_simplecar(orc_t *o, int on)
+{
+    o->simplecar = on ? 1 : 0;
+    return 0;
+}
+
 /* ---- VelPDcoeff, calculate.cu:3320-3466 --------------------------------- */
 static void matinv3(real *A) /* calculate.cu:3444-3466, Gauss-Jordan without pivoting */
 {
@@ -363,6 +375,10 @@ static void velpdcoeff(orc_t *o)
             real lambda = lam2mu - 2.0 * miu;
             real e11 = o->drv[XIX][G], e12 = o->drv[XIY][G], e13 = o->drv[XIZ][G];
             real e21 = o->drv[ETX][G], e22 = o->drv[ETY][G], e23 = o->drv[ETZ][G];
+            if (o->simplecar) { /* calculate.cu:3369-3375 */
+                e11 = 1; e12 = 0; e13 = 0;
+                e21 = 0; e22 = 1; e23 = 0;
+            }
             real e31 = o->drv[ZTX][G], e32 = o->drv[ZTY][G], e33 = o->drv[ZTZ][G];
             real A[9], B[9], C[9], tmp[9];
             A[0] = lam2mu * e31 * e31 + miu * (e32 * e32 + e33 * e33);
@@ -502,7 +518,72 @@ static void rhs(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int 
                 real DxiVx, DetVx, DztVx, DxiVy, DetVy, DztVy, DxiVz, DetVz, DztVz;
                 real DxiTxx, DetTxx, DztTxx, DxiTyy, DetTyy, DztTyy, DxiTzz, DetTzz, DztTzz;
                 real DxiTxy, DetTxy, DztTxy, DxiTxz, DetTxz, DztTxz, DxiTyz, DetTyz, DztTyz;
-                if (idz >= o->conindex) {
+                if (o->simplecar && idz >= o->conindex) {
+                    /* CalWavemix above ConIndex (calculate.cu:4339-4407): zeta
+                     * metrics only */
+                    ztx = o->drv[ZTX][G];
+                    zty = o->drv[ZTY][G];
+                    DxiVx = DxTxx * rho;
+                    DetVx = DyTxy * rho;
+                    DztVx = (ztx * DzTxx + zty * DzTxy + ztz * DzTxz) * rho;
+                    DxiVy = DxTxy * rho;
+                    DetVy = DyTyy * rho;
+                    DztVy = (ztx * DzTxy + zty * DzTyy + ztz * DzTyz) * rho;
+                    DxiVz = DxTxz * rho;
+                    DetVz = DyTyz * rho;
+                    DztVz = (ztx * DzTxz + zty * DzTyz + ztz * DzTzz) * rho;
+                    DxiTxx = lam2mu * DxVx;
+#if PLANT(18) /* lam2mu in place of lambda in the eta term of Txx (calculate.cu:4381) */
+                    DetTxx = lam2mu * DyVy;
+#else
+                    DetTxx = lambda * DyVy;
+#endif
+                    DztTxx = lam2mu * ztx * DzVx + lambda * zty * DzVy + lambda * ztz * DzVz;
+                    DxiTyy = lambda * DxVx;
+                    DetTyy = lam2mu * DyVy;
+                    DztTyy = lambda * ztx * DzVx + lam2mu * zty * DzVy + lambda * ztz * DzVz;
+                    DxiTzz = lambda * DxVx;
+                    DetTzz = lambda * DyVy;
+                    DztTzz = lambda * ztx * DzVx + lambda * zty * DzVy + lam2mu * ztz * DzVz;
+                    DxiTxy = DxVy * miu;
+                    DetTxy = DyVx * miu;
+                    DztTxy = (zty * DzVx + ztx * DzVy) * miu;
+                    DxiTxz = DxVz * miu;
+                    DetTxz = 0.0;
+                    DztTxz = (ztz * DzVx + ztx * DzVz) * miu;
+                    DxiTyz = 0.0;
+                    DetTyz = DyVz * miu;
+                    DztTyz = (ztz * DzVy + zty * DzVz) * miu;
+                } else if (o->simplecar) {
+                    /* CalWavemix below ConIndex (calculate.cu:4409-4456) */
+                    DxiVx = rho * DxTxx;
+                    DetVx = rho * DyTxy;
+                    DztVx = ztz * DzTxz * rho;
+                    DxiVy = rho * DxTxy;
+                    DetVy = rho * DyTyy;
+                    DztVy = ztz * DzTyz * rho;
+                    DxiVz = rho * DxTxz;
+                    DetVz = rho * DyTyz;
+                    DztVz = ztz * DzTzz * rho;
+                    DxiTxx = lam2mu * DxVx;
+                    DetTxx = lambda * DyVy;
+                    DztTxx = lambda * ztz * DzVz;
+                    DxiTyy = lambda * DxVx;
+                    DetTyy = lam2mu * DyVy;
+                    DztTyy = lambda * ztz * DzVz;
+                    DxiTzz = lambda * DxVx;
+                    DetTzz = lambda * DyVy;
+                    DztTzz = lam2mu * ztz * DzVz;
+                    DxiTxy = miu * DxVy;
+                    DetTxy = miu * DyVx;
+                    DztTxy = 0.0;
+                    DxiTxz = miu * DxVz;
+                    DetTxz = 0.0;
+                    DztTxz = ztz * DzVx * miu;
+                    DxiTyz = 0.0;
+                    DetTyz = miu * DyVz;
+                    DztTyz = ztz * DzVy * miu;
+                } else if (idz >= o->conindex) {
                     xiy = o->drv[XIY][G];
                     xiz = o->drv[XIZ][G];
                     etx = o->drv[ETX][G];
@@ -718,9 +799,19 @@ static void timg(orc_t *o, real *const *mW, real *const mA[3][NF], int Xvec, int
                     real VecTx[7], VecTy[7], VecTz[7];
                     for (int i = 0; i < 2 * LFD + 1; i++) {
                         long r = G - LFD * xs + i * xs;
-                        VecTx[i] = D[JAC][r] * (D[XIX][r] * T0[r] + D[XIY][r] * T1[r] + D[XIZ][r] * T2[r]);
+                        if (o->simplecar) /* CalTIMGmix (calculate.cu:5729-5740) */
+                            VecTx[i] = D[JAC][r] * T0[r];
+                        else
+                            VecTx[i] = D[JAC][r] * (D[XIX][r] * T0[r] + D[XIY][r] * T1[r] + D[XIZ][r] * T2[r]);
                         r = G - LFD * ys + i * ys;
-                        VecTy[i] = D[JAC][r] * (D[ETX][r] * T0[r] + D[ETY][r] * T1[r] + D[ETZ][r] * T2[r]);
+                        if (o->simplecar)
+#if PLANT(19) /* the x-row stress in place of the y-row one in CalTIMGmix's eta vector */
+                            VecTy[i] = D[JAC][r] * T0[r];
+#else
+                            VecTy[i] = D[JAC][r] * T1[r];
+#endif
+                        else
+                            VecTy[i] = D[JAC][r] * (D[ETX][r] * T0[r] + D[ETY][r] * T1[r] + D[ETZ][r] * T2[r]);
                         r = G - LFD + i;
 #if PLANT(6) /* Jacobian dropped from the zeta traction vector */
                         VecTz[i] = (D[ZTX][r] * T0[r] + D[ZTY][r] * T1[r] + D[ZTZ][r] * T2[r]);

@@ -58,6 +58,9 @@ int orc_set_pgv(orc_t *o, int on);
  * below it CalWave's Cartesian contraction applies (calculate.cu:3910-3956).
  * -1 unless 3 <= conindex <= nk: the three surface rows stay curvilinear. */
 int orc_set_conindex(orc_t *o, int conindex);
+/* the SimpleCar build (xi/eta metrics taken as the identity in CalWavemix,
+ * CalTIMGmix and VelPDcoeff, calculate.cu:1471-1480, 3362-3375) */
+int orc_set_simplecar(orc_t *o, int on);
 
 /* Zero the state and compute the free-surface closure matrices (VelPDcoeff). */
 int orc_init(orc_t *o);

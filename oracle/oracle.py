@@ -61,6 +61,7 @@ def _bind(lib, double):
         "orc_set_receivers": (ctypes.c_int, [VP, ctypes.c_int, IP]),
         "orc_set_pgv": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_set_conindex": (ctypes.c_int, [VP, ctypes.c_int]),
+        "orc_set_simplecar": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_init": (ctypes.c_int, [VP]),
         "orc_run": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_run_stages": (ctypes.c_int, [VP, ctypes.c_int]),
@@ -174,6 +175,9 @@ class Oracle:
     def set_conindex(self, conindex):
         self._check(self.lib.orc_set_conindex(self.h, int(conindex)), "set_conindex")
 
+    def set_simplecar(self, on=True):
+        self._check(self.lib.orc_set_simplecar(self.h, int(bool(on))), "set_simplecar")
+
     def init(self):
         self._check(self.lib.orc_init(self.h), "init")
 
@@ -192,6 +196,8 @@ class Oracle:
         self.set_pgv(sc.get("pgv", True))
         if sc.get("conindex") is not None:
             self.set_conindex(sc["conindex"])
+        if sc.get("simplecar"):
+            self.set_simplecar(True)
         self.init()
 
     def run(self, nsteps):

@@ -102,3 +102,31 @@ def test_set_hybrid_rejects_surface_rows():
             g.set_hybrid(bad)
     g.set_hybrid(sc["nk"])
     g.close()
+
+
+@pytest.mark.parametrize("conindex", [None, 3 + 10])
+def test_simplecar_build_matches_oracle(conindex):
+    """The reference's SimpleCar build (xi/eta metrics taken as the identity
+    in CalWavemix, CalTIMGmix and VelPDcoeff) on a sheared grid, where it
+    differs from the curvilinear run: engine and oracle value for value, in
+    FP32 (constant-horizontal-metrics kernel) and FP64."""
+    for double in (False, True):
+        sc = _scenario(65, 3 if conindex is None else conindex, shear=True, nt=20)
+        if conindex is None:
+            del sc["conindex"]
+        sc["simplecar"] = True
+        g, o = _pair(sc, double=double)
+        assert g.metrics_specialized() == (not double)
+        g.run(20)
+        o.run(20)
+        _compare(g, o, f"SimpleCar conindex {conindex} {'FP64' if double else 'FP32'}")
+        g.close()
+
+
+def test_simplecar_must_precede_the_metrics():
+    sc = _scenario(66, 3, nt=2)
+    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    g.set_metrics(sc["metrics"])
+    with pytest.raises(cgfd3d.CgfdError):
+        g.set_simplecar(True)
+    g.close()

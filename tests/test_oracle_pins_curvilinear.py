@@ -545,10 +545,74 @@ def pin_hybrid_equals_curvilinear_on_identity_metrics(lib=None):
         assert np.array_equal(x, y), f"{COMPS[c]}: {np.count_nonzero(x != y)} values differ"
 
 
+XI_ETA_IDENTITY = (("xix", 1.0), ("xiy", 0.0), ("xiz", 0.0), ("etax", 0.0), ("etay", 1.0), ("etaz", 0.0))
+
+
+def _run_pair(sa, sb, lib, steps=8, seed=41):
+    """Two FP64 oracles from the same random state, run `steps` steps."""
+    a, b = make_oracle(sa, lib, double=True), make_oracle(sb, lib, double=True)
+    rng = np.random.default_rng(seed)
+    for c, st in enumerate(scenario.random_state(a.shape, rng)):
+        a.set_field(c, st.astype(np.float64))
+        b.set_field(c, st.astype(np.float64))
+    a.run(steps)
+    b.run(steps)
+    return a, b
+
+
+def _all_equal(a, b):
+    out = [np.array_equal(a.get_field(c), b.get_field(c)) for c in range(9)]
+    out += [np.array_equal(a.get_aux(ax, c, 0), b.get_aux(ax, c, 0)) for ax in range(3) for c in range(9)]
+    out += [np.array_equal(a.get_seismo(), b.get_seismo()), np.array_equal(a.get_pgv(), b.get_pgv())]
+    return all(out)
+
+
+def pin_simplecar_is_the_identity_xi_eta_metrics(lib=None):
+    """(h) The SimpleCar build restates CalWavemix (calculate.cu:4339-4456),
+    CalTIMGmix (5729-5740) and VelPDcoeff's constants (3369-3375).  On a
+    sheared grid, whose xi/eta metrics are far from the identity, it equals
+    value for value the curvilinear run of the same grid with its xi/eta
+    metrics replaced by 1 and 0 (products with 1 and sums with zeros are
+    exact), with and without a hybrid part, PML, surface and sources; and
+    it differs from the run with the grid's own metrics."""
+    for conindex in (None, 3 + 8):
+        sc = scenario.config_random(24, 22, 18, 8, [4, 4, 3, 3, 4, 0], seed=41, shear=True)
+        sc = {k: (v.astype(np.float64) if k in ("alpha", "beta", "rho") else v) for k, v in sc.items()}
+        if conindex:
+            sc["conindex"] = conindex
+        met = dict(sc["metrics"])
+        for nm, v in XI_ETA_IDENTITY:
+            met[nm] = np.full(met[nm].shape, v, met[nm].dtype)
+        a, b = _run_pair(dict(sc, simplecar=True), dict(sc, metrics=met), lib)
+        assert np.abs(a.get_field(3)).max() > 0
+        assert _all_equal(a, b), f"SimpleCar vs identity xi/eta metrics, conindex {conindex}"
+        _, c = _run_pair(dict(sc, simplecar=True), sc, lib)
+        assert not np.array_equal(a.get_field(3), c.get_field(3))
+
+
+def pin_simplecar_equals_curvilinear_on_stretched_grids(lib=None):
+    """On a grid stretched only vertically under a topography (uniform x/y
+    spacing, as every vmap grid), the discrete xi/eta metrics are exactly 1
+    and 0, so the SimpleCar build and the curvilinear run agree value for
+    value, hybrid or not."""
+    for conindex in (None, 3 + 8):
+        sc = scenario.config_random(24, 22, 18, 8, [4, 4, 3, 3, 4, 0], seed=43, shear=False)
+        sc = {k: (v.astype(np.float64) if k in ("alpha", "beta", "rho") else v) for k, v in sc.items()}
+        for nm, v in XI_ETA_IDENTITY:
+            assert (sc["metrics"][nm][3:-3, 3:-3, 3:-3] == v).all(), nm
+        if conindex:
+            sc["conindex"] = conindex
+        a, b = _run_pair(dict(sc, simplecar=True), sc, lib, seed=43)
+        assert np.abs(a.get_field(0)).max() > 0
+        assert _all_equal(a, b), f"SimpleCar vs curvilinear, conindex {conindex}"
+
+
 PINS = {
     "affine_linear_stage": pin_affine_linear_stage,
     "hybrid_cartesian_branch": pin_hybrid_cartesian_branch,
     "hybrid_equals_curvilinear_on_identity_metrics": pin_hybrid_equals_curvilinear_on_identity_metrics,
+    "simplecar_is_the_identity_xi_eta_metrics": pin_simplecar_is_the_identity_xi_eta_metrics,
+    "simplecar_equals_curvilinear_on_stretched_grids": pin_simplecar_equals_curvilinear_on_stretched_grids,
     "fb_schedule": pin_fb_schedule,
     "closure_matrices": pin_closure_matrices,
     "source_magnitudes": pin_source_magnitudes,

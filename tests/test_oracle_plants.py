@@ -47,6 +47,10 @@ PLANTS = {
          pins_flat.pin_rk4_amplification),
     17: ("hybrid grid: lambda in place of lam2mu on the zz diagonal of the Cartesian branch (calculate.cu:3940)",
          pins_curv.pin_hybrid_cartesian_branch),
+    18: ("SimpleCar: lam2mu in place of lambda in the eta term of Txx (calculate.cu:4381)",
+         pins_curv.pin_simplecar_is_the_identity_xi_eta_metrics),
+    19: ("SimpleCar: the x-row stress in CalTIMGmix's eta traction vector (calculate.cu:5735)",
+         pins_curv.pin_simplecar_is_the_identity_xi_eta_metrics),
 }
 
 

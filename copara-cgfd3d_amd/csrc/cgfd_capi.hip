@@ -98,6 +98,7 @@ struct cgfd_ctx {
     bool use_graph = false, capturing = false;
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     int rows = 4, zchunks = 1;
+    bool zc_tuned = false; // zchunks chosen by cgfd_autotune or cgfd_set_tuning: fused batches keep it
     int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
     char *args_d = nullptr; // argument image (kCtxArgBytes: stage, traction-image and trace blocks)
     bool args_dirty = true;
@@ -190,6 +191,7 @@ static void default_tuning(cgfd_ctx *c)
     long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 24) / 25);
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
+    c->zc_tuned = false;
 }
 
 extern "C" {
@@ -1156,7 +1158,7 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
     for (int i = 0; i < n; i++) {
         const cgfd_ctx *c = ctxs[i];
         if (c->prof || c->variant != 0) return false;
-        most += batch_tiles(c) * std::max(1, c->g.nk / 2);
+        most += batch_tiles(c) * std::max({1, c->g.nk / 2, c->zc_tuned ? c->zchunks : 0});
         if (i == 0) continue;
         if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0))
             return false;
@@ -1169,7 +1171,8 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 // Workgroups of a batched stage launch: every grid sweeps its columns in
 // about the same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS
 // workgroups (tiles of 64 columns x stage_tile_rows rows) over all grids,
-// with at most ~25 and at least 2 planes per chunk per grid.
+// with at most ~25 and at least 2 planes per chunk per grid; a grid whose
+// z-chunk count was tuned (cgfd_autotune, cgfd_set_tuning) keeps its own.
 // zlen[c] is grid c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
@@ -1182,7 +1185,8 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, int *zlen, int *starts)
     starts[0] = 0;
     for (int i = 0; i < n; i++) {
         const int nk = ctxs[i]->g.nk;
-        const int zci = (int)std::min(std::max(zc, (long)(nk + 24) / 25), (long)std::max(1, nk / 2));
+        const int zci = ctxs[i]->zc_tuned ? std::max(1, std::min(ctxs[i]->zchunks, nk))
+                                          : (int)std::min(std::max(zc, (long)(nk + 24) / 25), (long)std::max(1, nk / 2));
         zlen[i] = (nk + zci - 1) / zci;
         starts[i + 1] = starts[i] + (int)(batch_tiles(ctxs[i]) * zci);
     }
@@ -1462,6 +1466,7 @@ int cgfd_autotune(cgfd_ctx *c, int trial_steps, int *chosen)
         if ((rc = cgfd_init(c)) != 0) break;
     }
     c->zchunks = best;
+    c->zc_tuned = true;
     c->args_dirty = true;
     // then eager stepping against graph replay of 4-step cycles (one warm-up
     // cycle captures the graph), when the run is long enough for both
@@ -1781,7 +1786,10 @@ int cgfd_set_tuning(cgfd_ctx *c, int zchunks, int rows)
     if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(c, CGFD_EINVAL, "rows must be 1, 2 or 4");
     if (zchunks < 0 || zchunks > c->g.nk) return fail(c, CGFD_EINVAL, "zchunks out of range");
     if (rows) c->rows = rows;
-    if (zchunks) c->zchunks = zchunks;
+    if (zchunks) {
+        c->zchunks = zchunks;
+        c->zc_tuned = true;
+    }
     c->args_dirty = true;
     return 0;
 }

@@ -641,6 +641,43 @@ def transform_traces(base, tr, g):
     return out
 
 
+# The seismogram head works on traces in reduced time: each station's record
+# starts at the direct P wave's arrival bound r / V_ALIGN (r the distance from
+# the source to the station, V_ALIGN above every P velocity of the medium, so
+# no arrival is cut) and is scaled by r / 1 km against geometric spreading.
+V_ALIGN = 6000.0
+
+
+def _station_geometry(base, src):
+    """Sample shifts [n, S] and distances [n, S] (m) from source rows to the
+    stations (source depth below the surface row, stations on it)."""
+    st = np.asarray(base["stations"], np.float64) * base["dh"]
+    src = np.asarray(src, np.float64)
+    dx = st[None, :, 0] - src[:, None, 0]
+    dy = st[None, :, 1] - src[:, None, 1]
+    r = np.sqrt(dx * dx + dy * dy + src[:, 2:3] ** 2)
+    shift = np.minimum(np.floor(r / V_ALIGN / base["dt"]).astype(np.int64), base["nt"] - 1)
+    return shift, np.maximum(r, base["dh"])
+
+
+def align_traces(base, src, tr, inverse=False):
+    """Station traces [n, S, 3, T] to reduced time and spreading-corrected
+    amplitude (or back, with inverse=True; the samples before the arrival
+    bound come back as zeros)."""
+    shift, r = _station_geometry(base, src)
+    tr = np.asarray(tr)
+    out = np.zeros_like(tr)
+    T = tr.shape[-1]
+    for n in range(tr.shape[0]):
+        for q in range(tr.shape[1]):
+            k, f = int(shift[n, q]), r[n, q] / 1000.0
+            if inverse:
+                out[n, q, :, k:] = tr[n, q, :, :T - k] / f
+            else:
+                out[n, q, :, :T - k] = tr[n, q, :, k:] * f
+    return out
+
+
 def fd_scenario(base, src):
     """The FD scenario dict of one source row."""
     dh = base["dh"]
@@ -749,7 +786,8 @@ def trace_error(pred, ref):
 class StationHead(torch.nn.Module):
     """Seismogram head: at each station, the trunk's latent and the input
     fields (source offset, distance, mechanism) through a three-layer MLP
-    (fused products) to the station's three velocity traces."""
+    (fused products) to the station's three velocity traces, in reduced
+    time and corrected for geometric spreading (align_traces)."""
 
     def __init__(self, width, cin, nsamp, hidden=256, seed=0):
         super().__init__()
@@ -820,7 +858,7 @@ class Surrogate:
         y = torch.from_numpy((y - self.mu) / self.sd).to(self.device)
         yt = None
         if traces is not None:
-            tm = traces / (10.0 ** src[:, 9])[:, None, None, None]
+            tm = align_traces(self.base, src, traces) / (10.0 ** src[:, 9])[:, None, None, None]
             self.tsd = float(np.sqrt(np.mean(np.square(tm, dtype=np.float64))))
             yt = torch.from_numpy((tm / self.tsd).astype(np.float32)).to(self.device)
         losses = []
@@ -897,7 +935,7 @@ class Surrogate:
             if touts is None or self.tsd is None:
                 raise SurrogateError("no trained seismogram head")
             tr = sum(touts) / len(touts) * self.tsd * m0[:, None, None, None]
-            res = res + (tr.cpu().numpy(),)
+            res = res + (align_traces(self.base, src, tr.cpu().numpy(), inverse=True),)
         return res
 
     def calibrate(self, params, pgv, target_err=0.05, quantile=0.5):

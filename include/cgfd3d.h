@@ -163,9 +163,10 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
  * precision, the default stage variant, the same metric mode and the same
  * step phase (step % 4), none timing its kernels.  Grid sizes, media, PML,
  * sources, receivers and snapshots may differ.  Such a batch runs on the
- * first context's stream, which waits for the others' queued work, with its
- * own z-chunk count (the contexts' tuning is left as it is); the other
- * streams wait for the batch.  Otherwise 0 (or CGFD_EINVAL for a null
+ * first context's stream, which waits for the others' queued work; a grid
+ * keeps the z-chunk count cgfd_autotune or cgfd_set_tuning gave it, the
+ * others get the batch's own (about 2048 workgroups over all grids); the
+ * other streams wait for the batch.  Otherwise 0 (or CGFD_EINVAL for a null
  * list). */
 int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few

@@ -188,3 +188,24 @@ def test_unfusable_lists_are_reported_and_still_run():
     cgfd3d.run_batch(batch, 11)
     _check_against_single(batch, scs, 12)
     other.close()
+
+
+def test_fused_batch_keeps_each_grids_tuned_chunks():
+    """Grids whose z-chunk count was set (cgfd_set_tuning) or timed
+    (cgfd_autotune) keep it inside a fused batch, next to a grid on the
+    batch's own count (including counts of 1 and of nk, which the batch's
+    rule never picks): results equal the single runs."""
+    nt = 12
+    scs = _scenarios(nt, sizes=[(40, 30, 20), (36, 22, 16), (33, 26, 18)])
+    batch = [_sim(sc) for sc in scs]
+    batch[0].set_tuning(zchunks=1)
+    batch[1].set_tuning(zchunks=16)
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 7)
+    cgfd3d.run_batch(batch, nt - 7)
+    _check_against_single(batch, scs, nt)
+    tuned = [_sim(sc) for sc in scs]
+    tuned[2].autotune(trial_steps=2)
+    assert cgfd3d.batch_fusable(tuned)
+    cgfd3d.run_batch(tuned, nt)
+    _check_against_single(tuned, scs, nt)

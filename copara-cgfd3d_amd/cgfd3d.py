@@ -6,6 +6,11 @@ inputs (metrics, media, CFS-PML tables, point sources, receivers) in the
 reference's [x][y][z]+3-ghost layout, the same stepping entry points and the
 same outputs (receiver traces, peak ground velocity, wavefield).  All compute
 runs in the HIP library; loading fails loudly if it has not been built.
+
+A process that also uses torch on the GPU (the surrogate) must import torch
+before this library is loaded: both bind libamdhip64.so.7, the first one
+loaded serves the whole process, and torch initialises only on the runtime it
+ships with (the engine runs on either).
 """
 import ctypes
 import os

@@ -4,6 +4,15 @@ import sys
 import numpy as np
 import pytest
 
+# torch before the engine library: torch and libcgfd3d.so both bind
+# libamdhip64.so.7, the first one loaded serves the whole process, and torch
+# initialises only on the runtime it ships with, so the surrogate's GPU tests
+# fail in a process that loaded the engine first
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for sub in ("copara-cgfd3d_amd", "oracle"):
     p = os.path.join(ROOT, sub)

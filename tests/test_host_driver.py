@@ -119,3 +119,30 @@ def test_driver_runs_the_reference_example_from_its_parameter_files(tmp_path):
         for name in h.variables:
             assert_same(f.variables[name].data, h.variables[name].data, f"seismo.nc {name}")
     g.close()
+
+
+@pytest.mark.gpu
+def test_driver_simplecar_option_matches_the_python_path(tmp_path):
+    """cgfd_child --conf ... --simplecar runs the reference's SimpleCar build
+    (xi/eta metrics taken as the identity) and gives the traces of the same
+    build prepared by seisconf.load(simplecar=True)."""
+    cgfd3d = pytest.importorskip("cgfd3d")
+    import seisconf
+    import seisconf_fixture as FX
+
+    if not os.path.exists(DRIVER):
+        pytest.fail("cgfd_child is not built (copara-cgfd3d_amd/build.sh)")
+    conf = FX.write_set(tmp_path / "set", nt=16, hybrid=1, conv="-1200.0")
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([DRIVER, "--conf", str(conf), "--simplecar", "--out", str(out)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sc = seisconf.load(str(conf), simplecar=True)
+    g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    g.setup(sc)
+    g.run(sc["nt"])
+    seis = g.get_seismo()
+    assert_same(np.fromfile(out / "traces.bin", np.float32).reshape(seis.shape), seis, "traces")
+    assert np.abs(seis).max() > 0
+    g.close()

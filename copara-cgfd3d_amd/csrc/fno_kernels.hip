@@ -6,6 +6,7 @@
 // of all parameters in one launch.
 //
 //   C[z] = act((C[z] +) A[z] * B[z] + bias + D[z]),  z = 0 .. batch-1
+//   (and, on request, C[z] again in bf16: the operand of the next product)
 //   A: M x K bf16, element (m, k) at A + z*sAz + m*sAm + k*sAk
 //   B: K x N bf16, element (k, n) at B + z*sBz + k*sBk + n*sBn
 //   C: M x N fp32, element (m, n) at C + z*sCz + m*sCm + n
@@ -43,6 +44,8 @@ struct GemmArgs {
     long sDz, sDm;
     float *pre;        // the value before the activation, laid out as C, or null
     int act;           // CGFD_FNO_ACT_*
+    unsigned short *Cb; // bf16 copy of the result, (m, n) at Cb + z*sEz + m*sEm + n, or null
+    long sEz, sEm;
 };
 
 // One lane's share of a 64 x 32 operand tile (rows r0.., k0..) whose element
@@ -86,6 +89,15 @@ __device__ __forceinline__ void park(unsigned short (*T)[BK + PAD], u16x8 v, int
     } else {
         *reinterpret_cast<u16x8 *>(&T[tid >> 2][(tid & 3) * 8]) = v;
     }
+}
+
+// bf16 of an fp32 value, rounded to nearest even (NaN kept quiet)
+__device__ __forceinline__ unsigned short to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
 }
 
 // GELU, erf form: x/2 (1 + erf(x/sqrt 2)), and its derivative
@@ -153,6 +165,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
     // lane l holds rows 4*(l/16)+i, column l%16 of each 16x16 block
     const float *D = g.D ? g.D + z * g.sDz : nullptr;
     float *pre = g.pre ? g.pre + z * g.sCz : nullptr;
+    unsigned short *Cb = g.Cb ? g.Cb + z * g.sEz : nullptr;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -170,6 +183,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
                     if (pre) pre[m * g.sCm + n] = v;
                     if (g.act == CGFD_FNO_ACT_GELU) v = gelu(v);
                     *p = v;
+                    if (Cb) Cb[m * g.sEm + n] = to_bf16(v);
                 }
             }
 }
@@ -193,14 +207,6 @@ int stage_mode(const void *p, long sZ, long sR, long sK)
     return STAGE_ANY;
 }
 
-// bf16 of an fp32 value, rounded to nearest even (NaN kept quiet)
-__device__ __forceinline__ unsigned short to_bf16(float f)
-{
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
 
 // gx may alias gy: each element is read before it is written, by one lane.
 __global__ void __launch_bounds__(256) gelu_bwd_kernel(const float *gy, const float *__restrict__ pre, float *gx,
@@ -217,22 +223,24 @@ __global__ void __launch_bounds__(256) gelu_bwd_kernel(const float *gy, const fl
 // One Adam step (no weight decay, no amsgrad), as torch.optim.Adam computes
 // it: m += (1-b1) (g - m); v = b2 v + (1-b2) g^2;
 // p -= (lr / bc1) m / (sqrt(v) / sqrt(bc2) + eps), with the scalars formed
-// in double and rounded once to float.  Block b updates chunk b: elements
+// in double and rounded once to float; with clear set the gradient is zeroed
+// as it is read.  Block b updates chunk b: elements
 // [off, off + CGFD_FNO_ADAM_CHUNK) of tensor t, with (t, off) = chunks[2b],
 // chunks[2b+1] and the tensor's (p, g, m, v, n) = tensors[5t .. 5t+4].
 __global__ void __launch_bounds__(256) adam_kernel(const long long *__restrict__ tensors,
                                                    const long long *__restrict__ chunks, float w1, float b2,
-                                                   float w2, float lr_bc1, float bc2_sqrt, float eps)
+                                                   float w2, float lr_bc1, float bc2_sqrt, float eps, int clear)
 {
     const long long t = chunks[2 * blockIdx.x], off = chunks[2 * blockIdx.x + 1];
     float *p = reinterpret_cast<float *>(tensors[5 * t]);
-    const float *g = reinterpret_cast<const float *>(tensors[5 * t + 1]);
+    float *g = reinterpret_cast<float *>(tensors[5 * t + 1]);
     float *m = reinterpret_cast<float *>(tensors[5 * t + 2]);
     float *v = reinterpret_cast<float *>(tensors[5 * t + 3]);
     const long long n = tensors[5 * t + 4];
     const long long end = off + CGFD_FNO_ADAM_CHUNK < n ? off + CGFD_FNO_ADAM_CHUNK : n;
     for (long long i = off + threadIdx.x; i < end; i += blockDim.x) {
         const float gi = g[i];
+        if (clear) g[i] = 0.0f; // the gradient is read once: the next backward accumulates into zeros
         const float mi = m[i] + w1 * (gi - m[i]);
         const float vi = v[i] * b2 + w2 * (gi * gi);
         m[i] = mi;
@@ -246,7 +254,7 @@ __global__ void __launch_bounds__(256) adam_kernel(const long long *__restrict__
 extern "C" int cgfd_fno_gemm_bf16_ex(const void *A, long sAz, long sAm, long sAk, const void *B, long sBz, long sBk,
                                      long sBn, float *C, long sCz, long sCm, int M, int N, int K, int batch,
                                      int accumulate, const float *bias, const float *D, long sDz, long sDm,
-                                     float *pre, int act, void *stream)
+                                     float *pre, int act, void *C_bf16, long sEz, long sEm, void *stream)
 {
     if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || batch <= 0) return CGFD_FNO_EINVAL;
     if (act != CGFD_FNO_ACT_NONE && act != CGFD_FNO_ACT_GELU) return CGFD_FNO_EINVAL;
@@ -254,7 +262,7 @@ extern "C" int cgfd_fno_gemm_bf16_ex(const void *A, long sAz, long sAm, long sAk
     if (tm > 65535 || batch > 65535 || tn > 0x7fffffffL) return CGFD_FNO_EINVAL;
     GemmArgs g{(const unsigned short *)A, (const unsigned short *)B, C, sAz, sAm, sAk, sBz, sBk, sBn, sCz, sCm,
                M, N, K, accumulate ? 1 : 0, stage_mode(A, sAz, sAm, sAk), stage_mode(B, sBz, sBn, sBk),
-               bias, D, sDz, sDm, pre, act};
+               bias, D, sDz, sDm, pre, act, (unsigned short *)C_bf16, sEz, sEm};
     const dim3 grid((unsigned)tn, (unsigned)tm, (unsigned)batch);
     const hipStream_t s = (hipStream_t)stream;
     if (g.amode == STAGE_K) launch_b<STAGE_K>(g, grid, s);
@@ -268,7 +276,7 @@ extern "C" int cgfd_fno_gemm_bf16(const void *A, long sAz, long sAm, long sAk, c
                                   int accumulate, void *stream)
 {
     return cgfd_fno_gemm_bf16_ex(A, sAz, sAm, sAk, B, sBz, sBk, sBn, C, sCz, sCm, M, N, K, batch, accumulate,
-                                 nullptr, nullptr, 0, 0, nullptr, CGFD_FNO_ACT_NONE, stream);
+                                 nullptr, nullptr, 0, 0, nullptr, CGFD_FNO_ACT_NONE, nullptr, 0, 0, stream);
 }
 
 extern "C" int cgfd_fno_gelu_bwd(const float *gy, const float *pre, float *gx, void *gx_bf16, long n, void *stream)
@@ -282,7 +290,7 @@ extern "C" int cgfd_fno_gelu_bwd(const float *gy, const float *pre, float *gx, v
 }
 
 extern "C" int cgfd_fno_adam(const long long *tensors, const long long *chunks, int nchunks, double lr,
-                             double beta1, double beta2, double eps, long long step, void *stream)
+                             double beta1, double beta2, double eps, long long step, int clear_grad, void *stream)
 {
     if (!tensors || !chunks || nchunks < 0 || step < 1 || !(lr >= 0.0) || !(eps > 0.0) || !(beta1 >= 0.0) ||
         !(beta1 < 1.0) || !(beta2 >= 0.0) || !(beta2 < 1.0))
@@ -292,6 +300,6 @@ extern "C" int cgfd_fno_adam(const long long *tensors, const long long *chunks, 
     const double bc2 = 1.0 - __builtin_pow(beta2, (double)step);
     hipLaunchKernelGGL(adam_kernel, dim3((unsigned)nchunks), dim3(256), 0, (hipStream_t)stream, tensors, chunks,
                        (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2), (float)(lr / bc1),
-                       (float)__builtin_sqrt(bc2), (float)eps);
+                       (float)__builtin_sqrt(bc2), (float)eps, clear_grad ? 1 : 0);
     return hipGetLastError() == hipSuccess ? 0 : CGFD_FNO_EHIP;
 }

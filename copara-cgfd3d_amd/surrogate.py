@@ -60,9 +60,10 @@ def load_library():
         L, P, I = ctypes.c_long, ctypes.c_void_p, ctypes.c_int
         D, LL = ctypes.c_double, ctypes.c_longlong
         lib.cgfd_fno_gemm_bf16.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P]
-        lib.cgfd_fno_gemm_bf16_ex.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P, P, L, L, P, I, P]
+        lib.cgfd_fno_gemm_bf16_ex.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P, P, L, L, P, I,
+                                              P, L, L, P]
         lib.cgfd_fno_gelu_bwd.argtypes = [P, P, P, P, L, P]
-        lib.cgfd_fno_adam.argtypes = [P, P, I, D, D, D, D, LL, P]
+        lib.cgfd_fno_adam.argtypes = [P, P, I, D, D, D, D, LL, I, P]
         for fn in ("cgfd_fno_gemm_bf16", "cgfd_fno_gemm_bf16_ex", "cgfd_fno_gelu_bwd", "cgfd_fno_adam"):
             getattr(lib, fn).restype = I
         _LIB = lib
@@ -83,16 +84,18 @@ def _as3(t):
     return t if t is None or t.dim() == 3 else t.unsqueeze(0)
 
 
-def gemm(a, b, out, accumulate=False, bias=None, addend=None, pre=None, act=ACT_NONE):
+def gemm(a, b, out, accumulate=False, bias=None, addend=None, pre=None, act=ACT_NONE, out_bf16=None):
     """out[z] = act((out[z] +) a[z] @ b[z] + bias + addend[z]) on the matrix
-    cores, the value before the activation also stored into pre[z] when
-    given.  a: (batch,) M x K bf16, b: (batch,) K x N bf16, out: (batch,)
-    M x N fp32 with unit column stride; any non-negative strides.  bias: N
-    fp32 (contiguous); addend: (batch,) M x N fp32 with unit column stride;
-    pre: strided as out.  Runs on the current stream."""
+    cores, the value before the activation also stored into pre[z] and the
+    result again in bf16 into out_bf16[z] when given.  a: (batch,) M x K
+    bf16, b: (batch,) K x N bf16, out: (batch,) M x N fp32 with unit column
+    stride; any non-negative strides.  bias: N fp32 (contiguous); addend:
+    (batch,) M x N fp32 with unit column stride; pre: strided as out;
+    out_bf16: (batch,) M x N bf16 with unit column stride.  Runs on the
+    current stream."""
     if a.dim() == 2:
         a, b, out = a.unsqueeze(0), b.unsqueeze(0), out.unsqueeze(0)
-    addend, pre = _as3(addend), _as3(pre)
+    addend, pre, out_bf16 = _as3(addend), _as3(pre), _as3(out_bf16)
     if not (a.is_cuda and b.is_cuda and out.is_cuda):
         raise SurrogateError("gemm operands must be on the GPU")
     if a.dtype != torch.bfloat16 or b.dtype != torch.bfloat16 or out.dtype != torch.float32:
@@ -118,19 +121,25 @@ def gemm(a, b, out, accumulate=False, bias=None, addend=None, pre=None, act=ACT_
                             or pre.stride() != out.stride()
                             or not _span_ok(pre, [(e - 1) * s for e, s in zip((Z, M, N), pre.stride())])):
         raise SurrogateError("gemm pre must be an fp32 tensor shaped and strided as out")
+    if out_bf16 is not None and (not out_bf16.is_cuda or out_bf16.dtype != torch.bfloat16
+                                 or tuple(out_bf16.shape) != (Z, M, N) or out_bf16.stride(2) != 1
+                                 or min(out_bf16.stride()) < 0
+                                 or not _span_ok(out_bf16, [(e - 1) * s for e, s in zip((Z, M, N), out_bf16.stride())])):
+        raise SurrogateError("gemm out_bf16 must be a (batch,) M x N bf16 tensor with unit column stride")
     sa, sb, so = a.stride(), b.stride(), out.stride()
     lib = load_library()
     stream = torch.cuda.current_stream().cuda_stream
-    if bias is None and addend is None and pre is None and act == ACT_NONE:
+    if bias is None and addend is None and pre is None and act == ACT_NONE and out_bf16 is None:
         rc = lib.cgfd_fno_gemm_bf16(a.data_ptr(), sa[0], sa[1], sa[2], b.data_ptr(), sb[0], sb[1], sb[2],
                                     out.data_ptr(), so[0], so[1], M, N, K, Z, int(accumulate), stream)
     else:
         sd = addend.stride() if addend is not None else (0, 0, 1)
+        se = out_bf16.stride() if out_bf16 is not None else (0, 0, 1)
         rc = lib.cgfd_fno_gemm_bf16_ex(
             a.data_ptr(), sa[0], sa[1], sa[2], b.data_ptr(), sb[0], sb[1], sb[2], out.data_ptr(), so[0], so[1],
             M, N, K, Z, int(accumulate), None if bias is None else bias.data_ptr(),
             None if addend is None else addend.data_ptr(), sd[0], sd[1], None if pre is None else pre.data_ptr(),
-            int(act), stream)
+            int(act), None if out_bf16 is None else out_bf16.data_ptr(), se[0], se[1], stream)
     if rc != 0:
         raise SurrogateError(f"cgfd_fno_gemm_bf16 returned {rc}")
     return out
@@ -207,21 +216,28 @@ class _Linear(torch.autograd.Function):
     kernel that also writes the bf16 operand of the two gradient products."""
 
     @staticmethod
-    def forward(ctx, x, w, b, d, gelu):
+    def forward(ctx, x, w, b, d, gelu, xb, want_bf16):
         K = x.shape[1]
         pad = -K % 8
-        xb = torch.nn.functional.pad(x.to(torch.bfloat16), (0, pad)).contiguous()
+        if xb is None or pad:  # a producer's bf16 copy serves as the operand when no padding is needed
+            xb = torch.nn.functional.pad(x.to(torch.bfloat16), (0, pad))
+        xb = xb.contiguous()
         wb = torch.nn.functional.pad(w.to(torch.bfloat16), (0, 0, 0, pad)).contiguous()
         y = torch.empty(x.shape[0], w.shape[1], device=x.device, dtype=torch.float32)
         pre = torch.empty_like(y) if gelu else None
+        yb = torch.empty(y.shape, device=x.device, dtype=torch.bfloat16) if want_bf16 else None
         gemm(xb, wb, y, bias=b.contiguous(), addend=None if d is None else d.contiguous(), pre=pre,
-             act=ACT_GELU if gelu else ACT_NONE)
+             act=ACT_GELU if gelu else ACT_NONE, out_bf16=yb)
         ctx.save_for_backward(xb, wb, pre)
         ctx.k, ctx.gelu, ctx.has_d = K, gelu, d is not None
-        return y
+        if yb is None:
+            return y
+        ctx.mark_non_differentiable(yb)
+        ctx.set_materialize_grads(False)
+        return y, yb
 
     @staticmethod
-    def backward(ctx, gy):
+    def backward(ctx, gy, *unused):
         xb, wb, pre = ctx.saved_tensors
         gy = gy.contiguous()
         if ctx.gelu:
@@ -230,7 +246,7 @@ class _Linear(torch.autograd.Function):
             gz, gzb = gy, gy.to(torch.bfloat16)
         gx = torch.empty(xb.shape, device=gy.device, dtype=torch.float32)
         gemm(gzb, wb.t(), gx)
-        return gx[:, :ctx.k], _weight_grad(xb, gzb)[:ctx.k], gz.sum(0), (gz if ctx.has_d else None), None
+        return gx[:, :ctx.k], _weight_grad(xb, gzb)[:ctx.k], gz.sum(0), (gz if ctx.has_d else None), None, None, None
 
 
 class _Bmm(torch.autograd.Function):
@@ -261,9 +277,9 @@ class _LeftConst(torch.autograd.Function):
     the truncated DFTs of the spectral layers."""
 
     @staticmethod
-    def forward(ctx, x, a):
+    def forward(ctx, x, a, xb=None):
         Z, K, N = x.shape
-        xb = x.contiguous().to(torch.bfloat16)
+        xb = x.contiguous().to(torch.bfloat16) if xb is None else xb.contiguous()
         y = torch.empty(Z, a.shape[0], N, device=x.device, dtype=torch.float32)
         gemm(a.unsqueeze(0).expand(Z, -1, -1), xb, y)
         ctx.save_for_backward(a)
@@ -276,7 +292,7 @@ class _LeftConst(torch.autograd.Function):
         gyb = gy.contiguous().to(torch.bfloat16)
         gx = torch.empty(Z, a.shape[1], N, device=gy.device, dtype=torch.float32)
         gemm(a.t().unsqueeze(0).expand(Z, -1, -1), gyb, gx)
-        return gx, None
+        return gx, None, None
 
 
 _DFT = {}
@@ -332,8 +348,9 @@ class Adam:
     """Adam (torch.optim.Adam's update without weight decay or amsgrad) on
     the project's kernel: one launch updates every parameter.  Gradients
     live in preallocated .grad tensors that backward accumulates into and
-    zero_grad clears in place, so the launch's address table is built once
-    (and again only if a .grad tensor is replaced)."""
+    the kernel clears as it reads them, so zero_grad launches nothing and
+    the launch's address table is built once (and again only if a .grad
+    tensor is replaced)."""
 
     def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8):
         self.params = [p for p in params]
@@ -363,17 +380,17 @@ class Adam:
         return self._table[1:]
 
     def zero_grad(self):
+        """The step clears every gradient it reads; only a gradient that was
+        replaced or dropped since needs work here."""
         for p in self.params:
             if p.grad is None:
                 p.grad = torch.zeros_like(p)
-            else:
-                p.grad.zero_()
 
     def step(self, lr=None):
         rows, chunks, n = self._tables()
         self.steps += 1
         rc = load_library().cgfd_fno_adam(rows.data_ptr(), chunks.data_ptr(), n, self.lr if lr is None else float(lr),
-                                          self.beta1, self.beta2, self.eps, self.steps,
+                                          self.beta1, self.beta2, self.eps, self.steps, 1,
                                           torch.cuda.current_stream().cuda_stream)
         if rc != 0:
             raise SurrogateError(f"cgfd_fno_adam returned {rc}")
@@ -391,15 +408,18 @@ def matmul(x, w, backend):
     return _Matmul.apply(x, w) if backend == "mfma" else x @ w
 
 
-def linear(x, w, b, backend, d=None, gelu=False):
+def linear(x, w, b, backend, d=None, gelu=False, xb=None, want_bf16=False):
     """act(x @ w + b (+ d)): the fused kernel (backend "mfma") or FP32
-    PyTorch (backend "torch")."""
+    PyTorch (backend "torch").  xb: x already in bf16 (a producer's copy);
+    with want_bf16 the result comes with its own bf16 copy, (y, yb) (yb is
+    None on the torch backend)."""
     if backend == "mfma":
-        return _Linear.apply(x, w, b, d, gelu)
+        return _Linear.apply(x, w, b, d, gelu, xb, want_bf16)
     y = x @ w + b
     if d is not None:
         y = y + d
-    return torch.nn.functional.gelu(y) if gelu else y
+    y = torch.nn.functional.gelu(y) if gelu else y
+    return (y, None) if want_bf16 else y
 
 
 def bmm(x, w, backend):
@@ -436,9 +456,9 @@ class FNO2d(torch.nn.Module):
         self.p1_w, self.p1_b = lin(width, proj), torch.nn.Parameter(torch.zeros(proj))
         self.p2_w, self.p2_b = lin(proj, cout), torch.nn.Parameter(torch.zeros(cout))
 
-    def _spectral(self, v, l, backend):
+    def _spectral(self, v, l, backend, vb=None):
         if backend == "mfma":
-            return self._spectral_dft(v, l)
+            return self._spectral_dft(v, l, vb)
         B, H, W, C = v.shape
         m = self.modes
         ft = torch.fft.rfft2(v, dim=(1, 2))                                   # [B, H, W/2+1, C]
@@ -454,14 +474,16 @@ class FNO2d(torch.nn.Module):
         out[:, H - m:, :m] = y[:, m:]
         return torch.fft.irfft2(out, s=(H, W), dim=(1, 2))
 
-    def _spectral_dft(self, v, l):
+    def _spectral_dft(self, v, l, vb=None):
         """The spectral layer with the truncated transforms as products with
         the dft_matrices, on channels-last activations: no FFT and no
-        full-size transposes, only the kept modes are permuted."""
+        full-size transposes, only the kept modes are permuted.  vb: v's bf16
+        copy from the producing layer, if it made one."""
         B, H, W, C = v.shape
         m = self.modes
         fw, gh, gi, fi = dft_matrices(H, W, m, v.device)
-        y1 = _LeftConst.apply(v.reshape(B * H, W, C), fw)                    # [(b,h), (ri,kw), c]
+        y1 = _LeftConst.apply(v.reshape(B * H, W, C), fw,
+                              None if vb is None else vb.reshape(B * H, W, C))  # [(b,h), (ri,kw), c]
         y2 = _LeftConst.apply(y1.reshape(B, 2 * H, m * C), gh)               # [b, (ri,kh), (kw,c)]
         x = y2.reshape(B, 2, 2 * m, m, C).permute(2, 3, 0, 1, 4).reshape(2 * m * m, B, 2 * C)
         wre, wim = self.spec_re[l], self.spec_im[l]
@@ -472,24 +494,28 @@ class FNO2d(torch.nn.Module):
         return _LeftConst.apply(z1.reshape(B * H, 2 * m, C), fi).reshape(B, H, W, C)
 
     def trunk(self, x, backend="mfma"):
-        """The lifting and the Fourier layers: the latent [B, H, W, width]."""
+        """The lifting and the Fourier layers: the latent [B, H, W, width]
+        and its bf16 copy [B*H*W, width] (None on the torch backend), which
+        every product's epilogue writes for the next products."""
         B, H, W, _ = x.shape
         R = B * H * W
-        v = linear(x.reshape(R, -1), self.lift_w, self.lift_b, backend).reshape(B, H, W, self.width)
+        v, vb = linear(x.reshape(R, -1), self.lift_w, self.lift_b, backend, want_bf16=True)
         for l in range(self.layers):
-            sp = self._spectral(v, l, backend).reshape(R, self.width)
-            v = linear(v.reshape(R, self.width), self.pw_w[l], self.pw_b[l], backend, d=sp,
-                       gelu=l < self.layers - 1).reshape(B, H, W, self.width)
-        return v
+            sp = self._spectral(v.reshape(B, H, W, self.width), l, backend, vb).reshape(R, self.width)
+            v, vb = linear(v, self.pw_w[l], self.pw_b[l], backend, d=sp, gelu=l < self.layers - 1, xb=vb,
+                           want_bf16=True)
+        return v.reshape(B, H, W, self.width), vb
 
-    def head(self, v, backend="mfma"):
+    def head(self, v, backend="mfma", vb=None):
         """The two-layer projection of a latent to the output channels."""
         B, H, W, _ = v.shape
-        h = linear(v.reshape(B * H * W, self.width), self.p1_w, self.p1_b, backend, gelu=True)
-        return linear(h, self.p2_w, self.p2_b, backend).reshape(B, H, W, -1)
+        h, hb = linear(v.reshape(B * H * W, self.width), self.p1_w, self.p1_b, backend, gelu=True, xb=vb,
+                       want_bf16=True)
+        return linear(h, self.p2_w, self.p2_b, backend, xb=hb).reshape(B, H, W, -1)
 
     def forward(self, x, backend="mfma"):
-        return self.head(self.trunk(x, backend), backend)
+        v, vb = self.trunk(x, backend)
+        return self.head(v, backend, vb)
 
 
 # ---------------------------------------------------------------------------
@@ -832,8 +858,8 @@ class Surrogate:
         self.tta = False
 
     def _outputs(self, k, x):
-        v = self.nets[k].trunk(x)
-        maps = self.nets[k].head(v)
+        v, vb = self.nets[k].trunk(x)
+        maps = self.nets[k].head(v, vb=vb)
         return maps, (self.heads[k](v, x, self.si, self.sj) if self.heads else None)
 
     def fit(self, params, pgv, epochs=60, batch=16, lr=3e-3, log=None, augment=False, traces=None):

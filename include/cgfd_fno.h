@@ -39,10 +39,13 @@ int cgfd_fno_gemm_bf16(const void *A, long sAz, long sAm, long sAk, const void *
  * bias: N fp32 values added to every row, or null.  D: an M x N fp32 addend
  * with (m, n) at D + z*sDz + m*sDm + n, or null.  pre: where the value before
  * the activation is stored (same layout as C; what the activation's backward
- * needs), or null.  act: CGFD_FNO_ACT_NONE or CGFD_FNO_ACT_GELU. */
+ * needs), or null.  act: CGFD_FNO_ACT_NONE or CGFD_FNO_ACT_GELU.  C_bf16:
+ * where C is stored again rounded to bf16 (nearest even), (m, n) at C_bf16 +
+ * z*sEz + m*sEm + n (the next product's operand), or null. */
 int cgfd_fno_gemm_bf16_ex(const void *A, long sAz, long sAm, long sAk, const void *B, long sBz, long sBk, long sBn,
                           float *C, long sCz, long sCm, int M, int N, int K, int batch, int accumulate,
-                          const float *bias, const float *D, long sDz, long sDm, float *pre, int act, void *stream);
+                          const float *bias, const float *D, long sDz, long sDm, float *pre, int act, void *C_bf16,
+                          long sEz, long sEm, void *stream);
 
 /* Backward of GELU over n fp32 elements: gx = gy * GELU'(pre), and, when
  * gx_bf16 is not null, the same values rounded to bf16 (nearest even) there
@@ -55,9 +58,11 @@ int cgfd_fno_gelu_bwd(const float *gy, const float *pre, float *gx, void *gx_bf1
  * count).  chunks: device array of 2 int64 per block (tensor index, first
  * element, a multiple of CGFD_FNO_ADAM_CHUNK below the count), nchunks
  * blocks.  step counts from 1; the bias corrections 1 - beta^step, lr /
- * (1 - beta1^step) and sqrt(1 - beta2^step) are formed in double. */
+ * (1 - beta1^step) and sqrt(1 - beta2^step) are formed in double.  With
+ * clear_grad, each gradient element is set to zero once read (the next
+ * backward pass then accumulates into zeros, with no separate clearing). */
 int cgfd_fno_adam(const long long *tensors, const long long *chunks, int nchunks, double lr, double beta1,
-                  double beta2, double eps, long long step, void *stream);
+                  double beta2, double eps, long long step, int clear_grad, void *stream);
 
 #ifdef __cplusplus
 }

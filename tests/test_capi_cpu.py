@@ -95,9 +95,11 @@ def test_fno_library_exports_and_validates_without_the_gpu():
     # the fused-epilogue product, the GELU backward and Adam reject bad
     # arguments before any launch
     g = lib.cgfd_fno_gemm_bf16_ex
-    g.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P, P, L, L, P, I, P]
-    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 4, 1, 0, None, None, 0, 0, None, 7, None) == -1
-    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 0, 1, 0, None, None, 0, 0, None, 1, None) == -1
+    g.argtypes = [P, L, L, L, P, L, L, L, P, L, L, I, I, I, I, I, P, P, L, L, P, I, P, L, L, P]
+    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 4, 1, 0, None, None, 0, 0, None, 7, None, 0, 0,
+             None) == -1
+    assert g(dummy, 0, 1, 1, dummy, 0, 1, 1, dummy, 0, 1, 4, 4, 0, 1, 0, None, None, 0, 0, None, 1, None, 0, 0,
+             None) == -1
     b = lib.cgfd_fno_gelu_bwd
     b.argtypes = [P, P, P, P, L, P]
     assert b(None, dummy, dummy, None, 4, None) == -1
@@ -105,11 +107,11 @@ def test_fno_library_exports_and_validates_without_the_gpu():
     assert b(dummy, dummy, dummy, None, 0, None) == 0          # nothing to do
     a = lib.cgfd_fno_adam
     D, LL = ctypes.c_double, ctypes.c_longlong
-    a.argtypes = [P, P, I, D, D, D, D, LL, P]
-    assert a(dummy, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 0, None) == -1     # steps count from 1
-    assert a(dummy, dummy, 1, 1e-3, 1.0, 0.999, 1e-8, 1, None) == -1     # beta1 must be below 1
-    assert a(None, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 1, None) == -1
-    assert a(dummy, dummy, 0, 1e-3, 0.9, 0.999, 1e-8, 1, None) == 0      # no chunks
+    a.argtypes = [P, P, I, D, D, D, D, LL, I, P]
+    assert a(dummy, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 0, 1, None) == -1     # steps count from 1
+    assert a(dummy, dummy, 1, 1e-3, 1.0, 0.999, 1e-8, 1, 1, None) == -1     # beta1 must be below 1
+    assert a(None, dummy, 1, 1e-3, 0.9, 0.999, 1e-8, 1, 1, None) == -1
+    assert a(dummy, dummy, 0, 1e-3, 0.9, 0.999, 1e-8, 1, 1, None) == 0      # no chunks
 
 
 NC4_HEADER = os.path.join(ROOT, "include", "cgfd_nc4.h")

@@ -56,6 +56,7 @@ SIGNATURES = {
     "cgfd_set_pgv": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_set_hybrid": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_set_simplecar": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_share_medium": (ctypes.c_int, [_VP, _VP]),
     "cgfd_init": (ctypes.c_int, [_VP]),
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -316,6 +317,12 @@ class Simulation:
         planes); below it the Cartesian contraction applies."""
         self._check(self.lib.cgfd_set_hybrid(self.h, int(conindex)), "set_hybrid")
 
+    def share_medium(self, src):
+        """Use src's metrics and media (device arrays) instead of a copy,
+        before this context's own metrics and media.  The arrays stay until
+        the last context holding them is closed."""
+        self._check(self.lib.cgfd_share_medium(self.h, src.h), "share_medium")
+
     def set_simplecar(self, on=True):
         """The reference's SimpleCar build: xi/eta metrics taken as the
         identity (call before set_metrics)."""
@@ -328,8 +335,11 @@ class Simulation:
         """Load every input of a scenario dict (see scenario.py) and init."""
         if sc.get("simplecar"):
             self.set_simplecar(True)
-        self.set_metrics(sc["metrics"])
-        self.set_media(sc["alpha"], sc["beta"], sc["rho"])
+        if sc.get("medium_of") is not None:
+            self.share_medium(sc["medium_of"])
+        else:
+            self.set_metrics(sc["metrics"])
+            self.set_media(sc["alpha"], sc["beta"], sc["rho"])
         self.set_pml(sc["nabs"], sc["pml"])
         if sc.get("force_locs") is not None and len(sc["force_locs"]):
             self.set_forces(sc["force_locs"], sc["force_vec"], sc["force_stf"])

@@ -9,12 +9,14 @@
 #include "cgfd_kernels.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <utility>
 #include <vector>
@@ -60,6 +62,13 @@ struct cgfd_ctx {
     bool use_hmet = true;  // let the stage kernel exploit it
     int zcon = 0;          // hybrid grid: engine planes z < zcon use the Cartesian contraction
     bool simplecar = false; // the reference's SimpleCar build: xi/eta metrics taken as the identity
+    // the metric and media arrays (met, med), reference-counted: contexts that
+    // share one medium (cgfd_share_medium) hold the same block
+    struct Medium {
+        char *met = nullptr, *med = nullptr;
+        std::atomic<int> refs{1};
+    };
+    Medium *medium = nullptr;
 
     // point sources: host copies until init builds the device tables
     int nfrc = 0, nmnt = 0, nstf_f = 0, nstf_m = 0;
@@ -167,6 +176,31 @@ template <typename T> static void dfree_t(T *&p)
     p = nullptr;
 }
 
+// drop this context's reference to its metric and media arrays; the last
+// holder frees them
+static void release_medium(cgfd_ctx *c)
+{
+    if (!c->medium) { // set-up failed before the block was made
+        dfree(c->met);
+        dfree(c->med);
+    } else if (--c->medium->refs == 0) {
+        dfree(c->medium->met);
+        dfree(c->medium->med);
+        delete c->medium;
+    }
+    c->medium = nullptr;
+    c->met = c->med = nullptr;
+}
+
+static int own_medium(cgfd_ctx *c)
+{
+    c->medium = new (std::nothrow) cgfd_ctx::Medium();
+    if (!c->medium) return fail(c, CGFD_ENOMEM, "out of host memory");
+    c->medium->met = c->met;
+    c->medium->med = c->med;
+    return 0;
+}
+
 // F/B signs of stage k of step t (simulation.cc:264-641): x and y share a
 // sign that alternates with the stage parity and flips for t%4 in {2,3}; z
 // alternates with the stage parity and flips for t%4 in {1,2}.
@@ -255,6 +289,7 @@ int cgfd_create(const cgfd_dims *dims, int precision, int device, cgfd_ctx **out
     if ((rc = dalloc(c, &c->sets, (size_t)NSETS * NF * g.fs * c->esz)) ||
         (rc = dalloc(c, &c->met, (size_t)NDRV * g.fs * c->esz)) ||
         (rc = dalloc(c, &c->med, (size_t)4 * g.fs * c->esz)) ||
+        (rc = own_medium(c)) ||
         (rc = dalloc(c, &c->pml, (size_t)3 * (g.ni + g.nj + g.nk) * c->esz)) ||
         (rc = dalloc(c, &c->staging, (size_t)c->nref * c->esz)) ||
         (rc = dalloc(c, &c->timg, (size_t)9 * c->tstr * c->esz)) ||
@@ -342,8 +377,7 @@ void cgfd_destroy(cgfd_ctx *c)
     drop_graphs(c);
     dfree_t(c->d_step);
     dfree(c->sets);
-    dfree(c->met);
-    dfree(c->med);
+    release_medium(c);
     dfree(c->pml);
     dfree(c->staging);
     for (int a = 0; a < 3; a++) dfree(c->aux[a]);
@@ -415,6 +449,8 @@ extern "C" {
 int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv_in)
 {
     if (!c || !drv_in) return fail(c, CGFD_EINVAL, "null argument");
+    if (c->medium && c->medium->refs > 1)
+        return fail(c, CGFD_EINVAL, "the metrics are shared with another context (cgfd_share_medium)");
     HIPCHK(c, hipSetDevice(c->dev));
     for (int k = 0; k < NDRV; k++)
         if (!drv_in[k]) return fail(c, CGFD_EINVAL, "null metric array");
@@ -453,6 +489,8 @@ int cgfd_set_metrics(cgfd_ctx *c, const void *const *drv_in)
 int cgfd_set_media(cgfd_ctx *c, const void *alpha, const void *beta, const void *rho)
 {
     if (!c || !alpha || !beta || !rho) return fail(c, CGFD_EINVAL, "null argument");
+    if (c->medium && c->medium->refs > 1)
+        return fail(c, CGFD_EINVAL, "the media are shared with another context (cgfd_share_medium)");
     HIPCHK(c, hipSetDevice(c->dev));
     // alpha -> slot 0, beta -> slot 1, rho -> slot 3; then derive
     // lam2mu, miu, 1/rho in slots 0-2 on the GPU
@@ -650,6 +688,32 @@ int cgfd_snap_shape(const cgfd_ctx *c, int isnap, int *npoints, int *ntime)
     if (!c || isnap < 0 || isnap >= (int)c->snaps.size()) return CGFD_EINVAL;
     if (npoints) *npoints = c->snaps[isnap].npts;
     if (ntime) *ntime = c->snaps[isnap].ntime;
+    return 0;
+}
+
+int cgfd_share_medium(cgfd_ctx *c, const cgfd_ctx *src)
+{
+    if (!c || !src) return c ? fail(c, CGFD_EINVAL, "null argument") : CGFD_EINVAL;
+    if (c == src) return fail(c, CGFD_EINVAL, "a context cannot share its own medium");
+    if (src->dev != c->dev || src->prec != c->prec || src->g.ni != c->g.ni || src->g.nj != c->g.nj ||
+        src->g.nk != c->g.nk)
+        return fail(c, CGFD_EINVAL, "cgfd_share_medium needs a context on the same device, of the same precision "
+                                    "and grid size");
+    if (!src->have_metrics || !src->have_media)
+        return fail(c, CGFD_EINVAL, "the source context has no metrics or media yet");
+    if (c->have_metrics || c->have_media)
+        return fail(c, CGFD_EINVAL, "cgfd_share_medium must come before this context's own metrics and media");
+    HIPCHK(c, hipSetDevice(c->dev));
+    release_medium(c);
+    c->medium = src->medium;
+    ++c->medium->refs;
+    c->met = src->met;
+    c->med = src->med;
+    c->hmet = src->hmet;
+    c->simplecar = src->simplecar;
+    c->have_metrics = c->have_media = true;
+    c->inited = false;
+    c->args_dirty = true;
     return 0;
 }
 
@@ -1250,6 +1314,14 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
         starts = sv.data();
     }
     batch_layout(ctxs, n, zl, starts);
+    // grids that share one medium and have the same size and chunks run
+    // interleaved (workgroup = tile * n + grid): the grids' copies of one tile
+    // run together and read the shared metric and media lines from cache
+    bool interleave = n > 1;
+    for (int i = 1; i < n && interleave; i++)
+        interleave = ctxs[i]->met == c0->met && ctxs[i]->g.ni == c0->g.ni && ctxs[i]->g.nj == c0->g.nj &&
+                     ctxs[i]->g.nk == c0->g.nk && zl[i] == zl[0] &&
+                     starts[i + 1] - starts[i] == starts[1] - starts[0];
     const size_t sofs = (size_t)n * kCtxArgBytes;
     int max_nrecv = 0, ni = 0, nj = 0;
     long cols = 0;
@@ -1311,7 +1383,7 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             launch_timg<R>(sx, sz, ni, nj, cols, TimgArgs<R>{},
                            (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n, 4, c0->stream);
             launch_stage_batch<R>(sx, sz, k, hmet, hybrid, (const StageArgs<R> *)(img_d + k * kStageArgsBytes),
-                                  starts_d, n, starts[n], tinc, c0->stream);
+                                  starts_d, interleave ? -n : n, starts[n], tinc, c0->stream);
         }
         launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
         for (int i = 0; i < n; i++) {

@@ -92,6 +92,8 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 // argument block lies c * kCtxArgBytes after pa and it owns workgroups
 // starts[c] .. starts[c+1]-1 (starts in device memory, starts[nctx] =
 // nblocks); all grids share the step phase and the metric mode hmet.
+// nctx = -n: n grids of one size and z-chunk length on one medium, the n
+// copies of each tile run one after another on one XCD (starts unused).
 // rows of a stage-kernel tile (64 columns wide) in this build
 int stage_tile_rows(bool fp64);
 template <typename R>

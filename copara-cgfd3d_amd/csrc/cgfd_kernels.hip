@@ -700,11 +700,28 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     // they are used instead of being preloaded into (and spilled from) SGPRs.
     // A batched launch (nctx > 1) covers several grids: grid c owns
     // workgroups starts[c] .. starts[c+1]-1 and its block lies c *
-    // kCtxArgBytes after pa.  The search runs on scalar registers.
+    // kCtxArgBytes after pa.  The search runs on scalar registers.  With
+    // nctx = -n the n grids are identical and interleaved tile by tile.
     typedef const __attribute__((address_space(4))) StageArgs<R> *AP;
     typedef const __attribute__((address_space(4))) int *SP;
     int ctx = 0, nb = gridDim.x, id = blockIdx.x;
-    if (nctx > 1) {
+    if (nctx < 0) {
+        // n identical grids on one medium: workgroups are dealt round-robin
+        // to the 8 XCDs, and the n copies of one tile follow each other on
+        // one XCD, which reads the metric and media lines they share from
+        // its L2.  Tiles past the last multiple of 8 run grid after grid.
+        const int n = -nctx;
+        nb = gridDim.x / n;
+        const int M = (nb >> 3) * 8 * n;
+        if (id < M) {
+            const int k = id >> 3;
+            ctx = k % n;
+            id = (k / n) * 8 + (id & 7);
+        } else {
+            ctx = (id - M) % n;
+            id = (nb >> 3) * 8 + (id - M) / n;
+        }
+    } else if (nctx > 1) {
         const SP st = (SP)starts;
         int lo = 0, hi = nctx - 1;
         while (lo < hi) {
@@ -1719,7 +1736,7 @@ template <typename R>
 void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, const StageArgs<R> *pa, const int *starts,
                         int nctx, int nblocks, int tinc, hipStream_t s)
 {
-    if (nctx < 1 || nblocks < 1) return;
+    if (nctx == 0 || nctx == -1 || nblocks < 1) return;
     StageArgs<R> a{}; // only the metric mode and the hybrid flag steer a batched launch
     a.hmet = hmet;
     a.zcon = hybrid ? 1 : 0;

@@ -715,12 +715,15 @@ def fd_scenario(base, src):
     return sc
 
 
-def run_fd(base, params, group=32, log=None, rows=False, graph=True, traces=False):
+def run_fd(base, params, group=32, log=None, rows=False, graph=True, traces=False, share=True):
     """Surface PGV maps [n, 3, w, w] of the window of the scenarios
     (parameter rows, or source rows with rows=True), advanced `group` at a
     time with cgfd_run_batch (replaying captured step cycles when graph is
     set; the results are the same); `log` gets a line per group.  With
-    traces=True also the stations' velocity traces [n, S, 3, nt]."""
+    traces=True also the stations' velocity traces [n, S, 3, nt].  With
+    share, the grids of a group hold one device copy of the medium
+    (cgfd_share_medium) and the batch runs their copies of each tile
+    together; the results are the same."""
     import cgfd3d
 
     src = np.asarray(params, np.float64) if rows else source_rows(params)
@@ -732,6 +735,8 @@ def run_fd(base, params, group=32, log=None, rows=False, graph=True, traces=Fals
         batch = []
         for p in src[g0:g0 + group]:
             sc = fd_scenario(base, p)
+            if share and batch:
+                sc["medium_of"] = batch[0]
             s = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
             s.setup(sc)
             s.set_graph(graph)

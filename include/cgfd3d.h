@@ -125,6 +125,15 @@ int cgfd_set_hybrid(cgfd_ctx *ctx, int conindex);
  * Call before cgfd_set_metrics; on grids whose xi/eta metrics are the
  * identity (uniform x/y spacing) both builds give the same values. */
 int cgfd_set_simplecar(cgfd_ctx *ctx, int on);
+/* No reference counterpart: ctx uses src's metrics and media (device arrays)
+ * instead of a copy of its own, for many scenarios on one medium (the
+ * surrogate's training runs).  Both must be on one device, of one precision
+ * and grid size; src must have its metrics and media, ctx none yet.  The
+ * arrays are freed with the last context that holds them; while more than
+ * one does, none of them may set metrics or media.  A fused batch of such
+ * grids with one z-chunk length runs their copies of each tile together, so
+ * the shared lines are read from cache. */
+int cgfd_share_medium(cgfd_ctx *ctx, const cgfd_ctx *src);
 
 /* VelCoeff (calculate.cu:1336-1353) plus state reset: zero the wavefield,
  * auxiliary fields, traces and PGV; compute the traction-free closure

@@ -2,7 +2,8 @@
 one launch per kernel and stage give, value for value, the traces, PGV,
 snapshots and final state of running each grid alone, with different
 sizes, media, PML widths, sources and receiver counts per grid, with
-single runs interleaved between batches, and in FP64.  Lists that cannot
+single runs interleaved between batches, in FP64, and with grids that
+share one device copy of their medium.  Lists that cannot
 share a launch (another step phase or metric mode, a repeated context,
 another stage variant, kernel timing) are reported as such and still run
 correctly on one stream per grid."""
@@ -209,3 +210,63 @@ def test_fused_batch_keeps_each_grids_tuned_chunks():
     assert cgfd3d.batch_fusable(tuned)
     cgfd3d.run_batch(tuned, nt)
     _check_against_single(tuned, scs, nt)
+
+
+def _on_one_medium(n, size, nt):
+    """n scenarios with their own sources, receivers and PML on the medium
+    (metrics and media) of the first."""
+    scs = _scenarios(nt, n=n, sizes=[size] * n)
+    for sc in scs[1:]:
+        for k in ("metrics", "alpha", "beta", "rho"):
+            sc[k] = scs[0][k]
+    return scs
+
+
+@pytest.mark.parametrize("n,size", [(8, (36, 30, 20)), (3, (70, 26, 18))])
+def test_batch_on_one_shared_medium_equals_single_runs(n, size):
+    """Grids that share one device copy of the medium (cgfd_share_medium)
+    run in a fused batch with the copies of each tile together; tile counts
+    that are and are not multiples of 8 (the interleave's tail).  Each equals
+    a single run with its own copy of the medium, value for value, also after
+    the context that loaded the medium is closed."""
+    nt = 10
+    scs = _on_one_medium(n, size, nt)
+    batch = [_sim(scs[0])]
+    for sc in scs[1:]:
+        batch.append(_sim(dict(sc, medium_of=batch[0])))
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 6)
+    _check_against_single(batch, scs, 6)
+    first = batch.pop(0)
+    first.close()  # the others still hold the medium
+    cgfd3d.run_batch(batch, nt - 6)
+    _check_against_single(batch, scs[1:], nt)
+
+
+def test_shared_medium_refuses_what_would_change_it():
+    scs = _on_one_medium(2, (36, 30, 20), 4)
+    a = _sim(scs[0])
+    b = cgfd3d.Simulation(36, 30, 20, scs[1]["dh"], scs[1]["dt"], 4)
+    b.share_medium(a)
+    for g in (a, b):
+        with pytest.raises(cgfd3d.CgfdError, match="shared"):
+            g.set_metrics(scs[0]["metrics"])
+        with pytest.raises(cgfd3d.CgfdError, match="shared"):
+            g.set_media(scs[0]["alpha"], scs[0]["beta"], scs[0]["rho"])
+    with pytest.raises(cgfd3d.CgfdError, match="before"):
+        b.share_medium(a)  # b has them now
+    other = cgfd3d.Simulation(36, 30, 21, scs[1]["dh"], scs[1]["dt"], 4)
+    with pytest.raises(cgfd3d.CgfdError, match="grid size"):
+        other.share_medium(a)
+    fp64 = cgfd3d.Simulation(36, 30, 20, scs[1]["dh"], scs[1]["dt"], 4, precision=cgfd3d.FP64)
+    with pytest.raises(cgfd3d.CgfdError, match="precision"):
+        fp64.share_medium(a)
+    empty = cgfd3d.Simulation(36, 30, 20, scs[1]["dh"], scs[1]["dt"], 4)
+    fresh = cgfd3d.Simulation(36, 30, 20, scs[1]["dh"], scs[1]["dt"], 4)
+    with pytest.raises(cgfd3d.CgfdError, match="no metrics"):
+        fresh.share_medium(empty)
+    b.close()
+    # a holds the medium alone again and may replace it
+    a.set_metrics(scs[0]["metrics"])
+    for g in (a, other, fp64, empty, fresh):
+        g.close()

@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay captured 4-step cycles (cgfd_set_graph)")
     ap.add_argument("--variant", type=int, default=0,
                     help="stage kernel: 0 = LDS-staged planes, 1 = direct loads, 2 = split velocity/stress")
+    ap.add_argument("--tile-width", type=int, default=0, choices=[0, 32, 64],
+                    help="tile width of the LDS stage kernel (0: the one the grid's rows pick)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed steps, run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -537,6 +539,7 @@ def main():
     log(f"workload {sc['name']} built in {time.perf_counter() - t0:.1f} s")
     sim = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
     sim.set_variant(args.variant)
+    sim.set_tile_width(args.tile_width)
     if args.graph:
         sim.set_graph(True)
     if args.zchunks or args.rows:
@@ -571,6 +574,7 @@ def main():
     stage_ms = prof["stage_ms"] / max(prof["stage_n"], 1)
     bytes_stage = algorithmic_bytes(sc)
     bytes_loaded = sim.stage_bytes()
+    tile_width = sim.tile_width() if args.variant == 0 else None
     achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
     seis = sim.get_seismo()
     finite = bool(abs(seis).max() < float("inf"))
@@ -602,7 +606,8 @@ def main():
         leg = surrogate_study("192x192x128", args.sur_leg, 16, 16, 120, 64, 16, 16, True, 2)
 
     cfg = describe(sc)
-    cfg.update({"graph_replay": bool(args.graph), "stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
+    cfg.update({"graph_replay": bool(args.graph), "tile_width": tile_width,
+                "stage_kernel_ms": round(stage_ms, 4), "timg_kernel_ms": round(prof["timg_ms"] / max(prof["timg_n"], 1), 4),
                 "bytes_per_stage_model": bytes_stage, "bytes_per_stage_loaded": bytes_loaded, "fp64_oracle": fp64,
                 "oracle_parity": parity, "surrogate_leg": leg,
                 "hbm_frac_of_loaded_bytes": round(bytes_loaded / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -57,6 +57,8 @@ SIGNATURES = {
     "cgfd_set_hybrid": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_set_simplecar": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_share_medium": (ctypes.c_int, [_VP, _VP]),
+    "cgfd_set_tile_width": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "cgfd_tile_width": (ctypes.c_int, [_VP]),
     "cgfd_init": (ctypes.c_int, [_VP]),
     "cgfd_run": (ctypes.c_int, [_VP, ctypes.c_int]),
     "cgfd_run_stages": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -441,6 +443,14 @@ class Simulation:
 
     def set_variant(self, variant):
         self._check(self.lib.cgfd_set_variant(self.h, int(variant)), "set_variant")
+
+    def set_tile_width(self, width):
+        """Tile width of the default stage kernel: 64, 32 or 0 (the one that
+        leaves fewer lanes idle on this grid's rows)."""
+        self._check(self.lib.cgfd_set_tile_width(self.h, int(width)), "set_tile_width")
+
+    def tile_width(self):
+        return int(self.lib.cgfd_tile_width(self.h))
 
     # ---- NetCDF outputs (seismo.nc, snap_NNN.nc, peakvel.nc) ---------------
     def write_seismo(self, path, tinv=1, pnum=None, lnum=None, posx=None, posy=None, posz=None):

@@ -108,6 +108,7 @@ struct cgfd_ctx {
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     int rows = 4, zchunks = 1;
     bool zc_tuned = false; // zchunks chosen by cgfd_autotune or cgfd_set_tuning: fused batches keep it
+    int tile_w = 0;        // LDS kernel tile width set by cgfd_set_tile_width or cgfd_autotune (0: by the row length)
     int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
     char *args_d = nullptr; // argument image (kCtxArgBytes: stage, traction-image and trace blocks)
     bool args_dirty = true;
@@ -218,10 +219,31 @@ static void fb_signs(int t, int k, int *sx, int *sz)
 // launch's tail short (600x600x300: 6.07 ms per stage with 2 chunks of 150
 // planes, 5.59 ms with 10 of 30, 5.30 ms with 12 of 25 after later changes;
 // 200x200x150 is fastest at 11 chunks of 14 planes).
+// Tile width of the LDS stage kernel: the one set or tuned, else 32 when a
+// row of ni cells takes over 8 % more lanes in 64-column tiles than in
+// 32-column ones.  Measured per stage: 80 cells (128 vs 96 lanes) 13 % and
+// 200 cells (256 vs 224) 10 % faster at 32; 600 cells (640 vs 608) 0.5 %
+// and 64 or 192 cells (no idle lanes) 1-16 % slower, so a 32 x 8 tile costs
+// about 6 % more per lane than a 64 x 4 one.  FP64 runs 64.
+static bool narrow_pays(long lanes64, long lanes32) { return lanes64 * 100 > lanes32 * 108; }
+static int tile_width(const cgfd_ctx *c)
+{
+    if (c->prec == CGFD_FP64) return 64;
+    if (c->tile_w) return c->tile_w;
+    return narrow_pays((c->g.ni + 63) / 64 * 64, (c->g.ni + 31) / 32 * 32) ? 32 : 64;
+}
+
+// LDS-kernel tiles of one z plane at tile width tw
+static long plane_tiles(const cgfd_ctx *c, int tw)
+{
+    const int tr = stage_tile_rows(c->prec == CGFD_FP64, tw);
+    return (long)((c->g.ni + tw - 1) / tw) * ((c->g.nj + tr - 1) / tr);
+}
+
 static void default_tuning(cgfd_ctx *c)
 {
     c->rows = 4;
-    const long tiles = (long)((c->g.ni + 63) / 64) * ((c->g.nj + c->rows - 1) / c->rows);
+    const long tiles = plane_tiles(c, tile_width(c));
     long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 24) / 25);
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
@@ -973,6 +995,7 @@ template <typename R> static StageArgs<R> make_stage_args(const cgfd_ctx *c, int
     sa.zlen = (c->g.nk + c->zchunks - 1) / c->zchunks;
     sa.hmet = (c->hmet && c->use_hmet) ? 1 : 0;
     sa.zcon = c->zcon;
+    sa.tile = tile_width(c);
     sa.step = c->d_step;
     return sa;
 }
@@ -1207,10 +1230,25 @@ int cgfd_set_graph(cgfd_ctx *c, int on)
 // precision, the LDS stage variant with the same metric mode, and the step
 // phase currT % 4 (which fixes the forward/backward pattern of every
 // stage), and no context is timing its kernels.  Grid sizes may differ.
-static long batch_tiles(const cgfd_ctx *c)
+// The tile width of a fused batch (one kernel for all grids): the width a
+// grid was given (cgfd_set_tile_width, cgfd_autotune) if any, else 32 when
+// 64-column tiles take over 8 % more tiles over all grids.  0 if two grids
+// were given different widths.
+static int batch_width(cgfd_ctx *const *ctxs, int n)
 {
-    const int ty = stage_tile_rows(c->prec == CGFD_FP64);
-    return (long)((c->g.ni + 63) / 64) * ((c->g.nj + ty - 1) / ty);
+    int given = 0;
+    long t64 = 0, t32 = 0;
+    for (int i = 0; i < n; i++) {
+        const cgfd_ctx *c = ctxs[i];
+        const int w = c->prec == CGFD_FP64 ? 64 : c->tile_w;
+        if (w) {
+            if (given && w != given) return 0;
+            given = w;
+        }
+        t64 += plane_tiles(c, 64);
+        t32 += c->prec == CGFD_FP64 ? plane_tiles(c, 64) : plane_tiles(c, 32);
+    }
+    return given ? given : (narrow_pays(t64, t32) ? 32 : 64);
 }
 
 static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
@@ -1218,11 +1256,13 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
     if (n < 2 || n > 4096) return false; // 9 traction-image parts per grid in a 65535 grid.z
     const cgfd_ctx *c0 = ctxs[0];
     auto hm = [](const cgfd_ctx *c) { return c->hmet && c->use_hmet; };
+    const int tw = batch_width(ctxs, n);
+    if (!tw) return false; // grids given different tile widths
     long most = 0; // workgroups at the most z chunks batch_layout can pick
     for (int i = 0; i < n; i++) {
         const cgfd_ctx *c = ctxs[i];
         if (c->prof || c->variant != 0) return false;
-        most += batch_tiles(c) * std::max({1, c->g.nk / 2, c->zc_tuned ? c->zchunks : 0});
+        most += plane_tiles(c, tw) * std::max({1, c->g.nk / 2, c->zc_tuned ? c->zchunks : 0});
         if (i == 0) continue;
         if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0))
             return false;
@@ -1234,17 +1274,17 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 
 // Workgroups of a batched stage launch: every grid sweeps its columns in
 // about the same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS
-// workgroups (tiles of 64 columns x stage_tile_rows rows) over all grids,
-// with at most ~25 and at least 2 planes per chunk per grid; a grid whose
-// z-chunk count was tuned (cgfd_autotune, cgfd_set_tuning) keeps its own.
-// zlen[c] is grid c's chunk length, starts[c] its first workgroup.
+// workgroups (tiles of the batch's width tw) over all grids, with at most
+// ~25 and at least 2 planes per chunk per grid; a grid whose z-chunk count
+// was tuned (cgfd_autotune, cgfd_set_tuning) keeps its own.  zlen[c] is grid
+// c's chunk length, starts[c] its first workgroup.
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
 #endif
-static void batch_layout(cgfd_ctx *const *ctxs, int n, int *zlen, int *starts)
+static void batch_layout(cgfd_ctx *const *ctxs, int n, int tw, int *zlen, int *starts)
 {
     long tiles = 0;
-    for (int i = 0; i < n; i++) tiles += batch_tiles(ctxs[i]);
+    for (int i = 0; i < n; i++) tiles += plane_tiles(ctxs[i], tw);
     const long zc = std::max(1L, (CGFD_BATCH_WORKGROUPS + tiles - 1) / tiles);
     starts[0] = 0;
     for (int i = 0; i < n; i++) {
@@ -1252,7 +1292,7 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, int *zlen, int *starts)
         const int zci = ctxs[i]->zc_tuned ? std::max(1, std::min(ctxs[i]->zchunks, nk))
                                           : (int)std::min(std::max(zc, (long)(nk + 24) / 25), (long)std::max(1, nk / 2));
         zlen[i] = (nk + zci - 1) / zci;
-        starts[i + 1] = starts[i] + (int)(batch_tiles(ctxs[i]) * zci);
+        starts[i + 1] = starts[i] + (int)(plane_tiles(ctxs[i], tw) * zci);
     }
 }
 
@@ -1313,7 +1353,8 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
         zl = zv.data();
         starts = sv.data();
     }
-    batch_layout(ctxs, n, zl, starts);
+    const int tw = batch_width(ctxs, n); // nonzero: the list is fusable
+    batch_layout(ctxs, n, tw, zl, starts);
     // grids that share one medium and have the same size and chunks run
     // interleaved (workgroup = tile * n + grid): the grids' copies of one tile
     // run together and read the shared metric and media lines from cache
@@ -1382,7 +1423,7 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
             launch_timg<R>(sx, sz, ni, nj, cols, TimgArgs<R>{},
                            (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n, 4, c0->stream);
-            launch_stage_batch<R>(sx, sz, k, hmet, hybrid, (const StageArgs<R> *)(img_d + k * kStageArgsBytes),
+            launch_stage_batch<R>(sx, sz, k, hmet, hybrid, tw, (const StageArgs<R> *)(img_d + k * kStageArgsBytes),
                                   starts_d, interleave ? -n : n, starts[n], tinc, c0->stream);
         }
         launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
@@ -1519,27 +1560,39 @@ int cgfd_autotune(cgfd_ctx *c, int trial_steps, int *chosen)
     int best = zc0;
     float best_ms = -1;
     int rc = 0;
+    // one warm-up step, then trial_steps timed, then the state is reset
+    auto trial = [&](float *ms) -> int {
+        c->args_dirty = true;
+        int r;
+        if ((r = cgfd_run_stages(c, 4)) != 0) return r; // warm the instantiations
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(c, CGFD_EHIP, "hipEventRecord");
+        if ((r = cgfd_run_stages(c, 4 * trial_steps)) != 0) return r;
+        if (hipEventRecord(e1, c->stream) != hipSuccess) return fail(c, CGFD_EHIP, "hipEventRecord");
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            return fail(c, CGFD_EHIP, "event timing failed");
+        return cgfd_init(c);
+    };
     for (int zc : cand) {
         c->zchunks = zc;
-        c->args_dirty = true;
-        if ((rc = cgfd_run_stages(c, 4)) != 0) break; // warm the instantiations
-        if ((rc = hipEventRecord(e0, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
-        if ((rc = cgfd_run_stages(c, 4 * trial_steps)) != 0) break;
-        if ((rc = hipEventRecord(e1, c->stream)) != hipSuccess) { rc = fail(c, CGFD_EHIP, "hipEventRecord"); break; }
         float ms = 0;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(c, CGFD_EHIP, "event timing failed");
-            break;
-        }
+        if ((rc = trial(&ms)) != 0) break;
         if (best_ms < 0 || ms < best_ms) {
             best_ms = ms;
             best = zc;
         }
-        if ((rc = cgfd_init(c)) != 0) break;
     }
     c->zchunks = best;
     c->zc_tuned = true;
     c->args_dirty = true;
+    // then the LDS kernel's other tile width (FP32) at those chunks
+    if (!rc && c->prec == CGFD_FP32 && c->variant == 0) {
+        const int w0 = tile_width(c);
+        c->tile_w = w0 == 64 ? 32 : 64;
+        float ms = 0;
+        rc = trial(&ms);
+        if (rc || !(ms < best_ms)) c->tile_w = w0;
+        c->args_dirty = true;
+    }
     // then eager stepping against graph replay of 4-step cycles (one warm-up
     // cycle captures the graph), when the run is long enough for both
     if (!rc && trial_steps >= 4 && c->d.nt >= 8) {
@@ -1851,6 +1904,20 @@ int cgfd_profile_read(cgfd_ctx *c, double *ms_stage, long *n_stage, double *ms_t
     if (n_timg) *n_timg = c->n_timg;
     return 0;
 }
+
+int cgfd_set_tile_width(cgfd_ctx *c, int width)
+{
+    if (!c) return CGFD_EINVAL;
+    if (width != 0 && width != 32 && width != 64) return fail(c, CGFD_EINVAL, "tile width must be 0, 32 or 64");
+    if (width == 32 && c->prec == CGFD_FP64) return fail(c, CGFD_EINVAL, "the FP64 kernels run 64-column tiles");
+    if (width != c->tile_w) {
+        c->tile_w = width;
+        c->args_dirty = true; // the next run rewrites the image and drops captured cycles
+    }
+    return 0;
+}
+
+int cgfd_tile_width(const cgfd_ctx *c) { return c ? tile_width(c) : CGFD_EINVAL; }
 
 int cgfd_set_tuning(cgfd_ctx *c, int zchunks, int rows)
 {

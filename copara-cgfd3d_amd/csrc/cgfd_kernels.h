@@ -42,6 +42,7 @@ template <typename R> struct StageArgs {
     int zlen;
     int hmet; // 1: horizontal metrics are the constants 1/0 (variant 0 loads only the zeta metrics)
     int zcon; // hybrid grid: planes z < zcon take the Cartesian contraction (0: none)
+    int tile; // LDS kernel tile width, 64 or 32 columns (host side: picks the kernel)
     const int *step; // device step counter: the current step currT (Tindex = 2*currT + tinc)
 };
 
@@ -94,11 +95,11 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 // nblocks); all grids share the step phase and the metric mode hmet.
 // nctx = -n: n grids of one size and z-chunk length on one medium, the n
 // copies of each tile run one after another on one XCD (starts unused).
-// rows of a stage-kernel tile (64 columns wide) in this build
-int stage_tile_rows(bool fp64);
+// rows of an LDS stage-kernel tile of tw columns (64 or 32; FP64 runs 64)
+int stage_tile_rows(bool fp64, int tw);
 template <typename R>
-void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, const StageArgs<R> *pa, const int *starts,
-                        int nctx, int nblocks, int tinc, hipStream_t s);
+void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, int tw, const StageArgs<R> *pa,
+                        const int *starts, int nctx, int nblocks, int tinc, hipStream_t s);
 // traction image: of one grid with its arguments av (pa null, nctx 1), or of
 // nctx grids whose blocks lie at kCtxArgBytes strides from pa (av unused);
 // ni x nj covers every grid's surface, cols is their total column count

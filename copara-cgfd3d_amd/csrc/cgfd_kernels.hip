@@ -691,7 +691,7 @@ template <typename R> constexpr int lds_ty() { return sizeof(R) == 4 ? LDS_TY : 
 #endif
 #define CGFD_LDS_ATTR __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CGFD_LDS_WAVES : 1, 8)))
 
-template <typename R, int SX, int SZ, int ST, int HM>
+template <typename R, int SX, int SZ, int ST, int HM, int TW>
 __global__ void __launch_bounds__(64 * lds_ty<R>()) CGFD_LDS_ATTR
 stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 {
@@ -735,7 +735,12 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     }
     const AP a0 = (AP)((const char *)pa + (long)ctx * kCtxArgBytes);
     const int tindex = 2 * *a0->step + tinc;
-    constexpr int TY = lds_ty<R>(), HX = 64 + 4, HY = TY + 4;
+    // A workgroup of TY wavefronts owns a tile of TW columns x TR rows: with
+    // TW = 64 a wavefront is one tile row, with TW = 32 it holds two (rows
+    // whose length leaves a short last 64-column tile waste fewer lanes).
+    static_assert(TW == 64 || TW == 32, "tile width");
+    constexpr int TY = lds_ty<R>(), HALVES = 64 / TW, TR = TY * HALVES;
+    constexpr int HX = TW + 4, HY = TR + 4;
     constexpr int O = SX > 0 ? 1 : 3; // halo cells below the tile in x and in y
     constexpr int ZOFF = SZ > 0 ? -1 : -3;
     constexpr int CI = SZ > 0 ? 1 : 3;
@@ -753,7 +758,10 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 
     const AP a = a0;
     const Geom g = geom_of(a);
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int lane = threadIdx.x, wv = threadIdx.y;
+    // which tile row of its wavefront the lane is in (blockDim.x is 64)
+    const int part = TW == 64 ? 0 : lane / TW;
+    const int tx = TW == 64 ? lane : lane % TW, ty = wv * HALVES + part;
 
     // Tile numbering.  Workgroups are dealt round-robin to the 8 XCDs, so
     // the identity map (default) gives every XCD a uniform sample of the
@@ -763,7 +771,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     // the bottom PML planes finish last); map 4 does that within groups of
     // 8*CGFD_TILE_GROUP consecutive tiles.  Both cut the L2 fetch traffic by
     // ~8 % on the 600x600x300 model yet run slower than the identity map.
-    const int ntx = (g.ni + 63) / 64, nty = (g.nj + TY - 1) / TY;
+    const int ntx = (g.ni + TW - 1) / TW, nty = (g.nj + TR - 1) / TR;
 #ifndef CGFD_TILE_MAP
 #define CGFD_TILE_MAP 1
 #endif
@@ -794,7 +802,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     const int L = xcd * q + min(xcd, r) + id / 8;
 #endif
     const int bx = L % ntx, by = (L / ntx) % nty, bz = L / (ntx * nty);
-    const int x0 = bx * 64, y0 = by * TY;
+    const int x0 = bx * TW, y0 = by * TR;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < g.ni && y < g.nj;
     const int z0 = bz * a->zlen;
@@ -806,27 +814,33 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     const unsigned esz = sizeof(R);
     const unsigned plb = (unsigned)(g.plane * esz); // bytes per plane
 
-    // Addresses.  x < ntx*64 <= nxp always; rows past the last guard row
+    // Addresses.  x < ntx*TW <= nxp always; rows past the last guard row
     // (y >= nyp, only in a ragged last tile) are clamped onto it (a zero row),
     // so every load stays inside its own plane.
     const int yc = min(y, g.nyp - 1);
-    // halo slots of this lane: one of the 4 halo rows (all 64 columns) for
-    // the first 4 wavefronts, and for lanes 0..4*TY-1 of every wavefront one
-    // of the 4 halo columns (the TY tile rows) of the fields hk, hk+TY, ...
-    // (each wavefront stages the x halo of NH of the 9 fields)
-    const int hk = tid >> 6;
-    const int hrow = SX > 0 ? (hk == 0 ? 0 : TY + hk) : (hk < 3 ? hk : TY + 3);
+    // Halo slots of this lane.  Halo rows: wavefront hk (the first 4) stages
+    // halo row hk (TW columns); with two tile rows per wavefront its lanes of
+    // part 0 take fields 0-4 of that row and those of part 1 fields 5-8, so
+    // a lane stages NR fields.  Halo columns: the 4*TR slots of one field
+    // (4 columns x TR rows) go to lanes q = lane % TW < 4*TR of one part;
+    // wavefront hk, part p stages fields hk + TY*(HALVES*j + p), j < NH.
+    // Slots past the last field reload field 8 and discard it, so every
+    // lane issues the same loads.
+    const int hk = wv;
+    const int hrow = SX > 0 ? (hk == 0 ? 0 : TR + hk) : (hk < 3 ? hk : TR + 3);
     const int hy = y0 - O + hrow, hxx = x0 + tx;
     const bool hv1 = hk < 4 && hy >= -3 && hy < g.nyp;
-    constexpr int NH = (NF + TY - 1) / TY;
-    const int hc2k = tx / TY, hr2 = tx % TY;
-    const int hcol = SX > 0 ? (hc2k == 0 ? 0 : 64 + hc2k) : (hc2k < 3 ? hc2k : 67);
+    constexpr int NR = (NF + HALVES - 1) / HALVES;
+    auto hrfield = [&](int j) { return min(part * NR + j, NF - 1); };
+    auto hrkeep = [&](int j) { return hk < 4 && part * NR + j < NF; };
+    constexpr int NH = (NF + TY * HALVES - 1) / (TY * HALVES);
+    const bool xlane = tx < 4 * TR;
+    const int hc2k = tx / TR, hr2 = tx % TR;
+    const int hcol = SX > 0 ? (hc2k == 0 ? 0 : TW + hc2k) : (hc2k < 3 ? hc2k : TW + 3);
     const int hx2 = x0 - O + hcol, hy2 = y0 + hr2;
-    const bool hv2 = tx < 4 * TY && hx2 >= -3 && hx2 < g.nxp && hy2 < g.nyp;
-    // field of x-halo slot j of this wavefront (wavefronts past the last
-    // field reload field 8 and discard it, so every wavefront issues NH loads)
-    auto hfield = [&](int j) { return min(hk + TY * j, NF - 1); };
-    auto hkeep = [&](int j) { return hk + TY * j < NF; };
+    const bool hv2 = xlane && hx2 >= -3 && hx2 < g.nxp && hy2 < g.nyp;
+    auto hfield = [&](int j) { return min(hk + TY * (HALVES * j + part), NF - 1); };
+    auto hkeep = [&](int j) { return xlane && hk + TY * (HALVES * j + part) < NF; };
     // byte offsets inside plane 0 (may be negative: added to a plane base);
     // invalid halo slots fall back to the lane's own cell
     const int rel_c = (yc * g.nxp + x) * (int)esz;
@@ -847,22 +861,22 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     // halos of the planes z0 .. z0+HD-1 go straight into their ring tiles
     for (int p = z0; p < z0 + HD; ++p) {
 #pragma unroll
-        for (int c = 0; c < NF; ++c) {
-            const R v1 = ldg(a->win + c * fs0, pbase(p) + rel_1);
-            if (hk < 4) lds[slot(p)][c][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
+        for (int j = 0; j < NR; ++j) {
+            const R v1 = ldg(a->win + hrfield(j) * fs0, pbase(p) + rel_1);
+            if (hrkeep(j)) lds[slot(p)][hrfield(j)][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
         }
 #pragma unroll
         for (int j = 0; j < NH; ++j) {
             const R v2 = ldg(a->win + hfield(j) * fs0, pbase(p) + rel_2);
-            if (tx < 4 * TY && hkeep(j)) lds[slot(p)][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? v2 : (R)0);
+            if (hkeep(j)) lds[slot(p)][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? v2 : (R)0);
         }
     }
 
-    R lead[NF], h1[NF], h2[NH];
+    R lead[NF], h1[NR], h2[NH];
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         lead[c] = ldg(a->win + c * fs0, pbase(z0 + D) + rel_c);
-        h1[c] = ldg(a->win + c * fs0, pbase(z0 + HD) + rel_1);
+        if (c < NR) h1[c] = ldg(a->win + hrfield(c) * fs0, pbase(z0 + HD) + rel_1);
     }
 #pragma unroll
     for (int j = 0; j < NH; ++j) h2[j] = ldg(a->win + hfield(j) * fs0, pbase(z0 + HD) + rel_2);
@@ -890,19 +904,19 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             lds[sc][c][O + ty][O + tx] = (LT)w[c][CI];
-            if (hk < 4) lds[sh][c][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
+            if (c < NR && hrkeep(c)) lds[sh][hrfield(c)][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
             w[c][4] = lead[c];
         }
 #pragma unroll
         for (int j = 0; j < NH; ++j)
-            if (tx < 4 * TY && hkeep(j)) lds[sh][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? h2[j] : (R)0);
+            if (hkeep(j)) lds[sh][hfield(j)][O + hr2][hcol] = (LT)(hv2 ? h2[j] : (R)0);
         __syncthreads();
         // next plane's lead (plane z+1+D <= nk+3) and halo (plane z+1+HD <=
         // nk+3) lie inside the 4 zero planes above the grid
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             lead[c] = ldg(al->win + c * fs, pbase(z + 1 + D) + rel_c);
-            h1[c] = ldg(al->win + c * fs, pbase(z + 1 + HD) + rel_1);
+            if (c < NR) h1[c] = ldg(al->win + hrfield(c) * fs, pbase(z + 1 + HD) + rel_1);
         }
 #pragma unroll
         for (int j = 0; j < NH; ++j) h2[j] = ldg(al->win + hfield(j) * fs, pbase(z + 1 + HD) + rel_2);
@@ -1684,21 +1698,38 @@ __global__ void snap_kernel(const R *fw, long fs, const long *lin, int npts, int
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// The LDS kernel at the tile width a.tile (32 or 64 columns; FP64 always 64).
+template <typename R, int SX, int SZ, int ST, int HM>
+static void launch_lds(const StageArgs<R> &a, const StageArgs<R> *pa, const int *starts, int nctx, int nblocks,
+                       int tinc, int zchunks, hipStream_t s)
+{
+    constexpr int ty = lds_ty<R>();
+    const dim3 block(64, ty);
+    if constexpr (sizeof(R) == 4) {
+        if (a.tile == 32) {
+            const int ntx = (a.g.ni + 31) / 32, nty = (a.g.nj + 2 * ty - 1) / (2 * ty);
+            const dim3 grid((unsigned)(nblocks > 0 ? nblocks : ntx * nty * zchunks));
+            stage_lds_kernel<R, SX, SZ, ST, HM, 32><<<grid, block, 0, s>>>(pa, starts, nctx, tinc);
+            return;
+        }
+    }
+    const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
+    const dim3 grid((unsigned)(nblocks > 0 ? nblocks : ntx * nty * zchunks));
+    stage_lds_kernel<R, SX, SZ, ST, HM, 64><<<grid, block, 0, s>>>(pa, starts, nctx, tinc);
+}
+
 template <typename R, int SX, int SZ, int ST>
 static void launch_stage_one(int variant, const StageArgs<R> &a, const StageArgs<R> *pa, const int *starts, int nctx,
                              int nblocks, int tinc, int rows, int zchunks, hipStream_t s)
 {
     if (variant == 0) {
-        constexpr int ty = lds_ty<R>();
-        const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + ty - 1) / ty;
-        const dim3 grid((unsigned)(nblocks > 0 ? nblocks : ntx * nty * zchunks));
         // mode bit 0: constant horizontal metrics; bit 1: a hybrid grid's
         // Cartesian planes
         switch ((a.hmet ? 1 : 0) | (a.zcon > 0 ? 2 : 0)) {
-        case 0: stage_lds_kernel<R, SX, SZ, ST, 0><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
-        case 1: stage_lds_kernel<R, SX, SZ, ST, 1><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
-        case 2: stage_lds_kernel<R, SX, SZ, ST, 2><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
-        default: stage_lds_kernel<R, SX, SZ, ST, 3><<<grid, dim3(64, ty), 0, s>>>(pa, starts, nctx, tinc); break;
+        case 0: launch_lds<R, SX, SZ, ST, 0>(a, pa, starts, nctx, nblocks, tinc, zchunks, s); break;
+        case 1: launch_lds<R, SX, SZ, ST, 1>(a, pa, starts, nctx, nblocks, tinc, zchunks, s); break;
+        case 2: launch_lds<R, SX, SZ, ST, 2>(a, pa, starts, nctx, nblocks, tinc, zchunks, s); break;
+        default: launch_lds<R, SX, SZ, ST, 3>(a, pa, starts, nctx, nblocks, tinc, zchunks, s); break;
         }
     } else if (variant == 2) {
         const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + SPLIT_TY - 1) / SPLIT_TY;
@@ -1733,13 +1764,14 @@ void launch_stage(int variant, int sx, int sz, int st, const StageArgs<R> &a, co
 }
 
 template <typename R>
-void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, const StageArgs<R> *pa, const int *starts,
-                        int nctx, int nblocks, int tinc, hipStream_t s)
+void launch_stage_batch(int sx, int sz, int st, int hmet, int hybrid, int tw, const StageArgs<R> *pa,
+                        const int *starts, int nctx, int nblocks, int tinc, hipStream_t s)
 {
     if (nctx == 0 || nctx == -1 || nblocks < 1) return;
-    StageArgs<R> a{}; // only the metric mode and the hybrid flag steer a batched launch
+    StageArgs<R> a{}; // only the metric mode, the hybrid flag and the tile width steer a batched launch
     a.hmet = hmet;
     a.zcon = hybrid ? 1 : 0;
+    a.tile = tw;
     if (sx > 0 && sz > 0) launch_stage_sxz<R, 1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
     else if (sx > 0) launch_stage_sxz<R, 1, -1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
     else if (sz > 0) launch_stage_sxz<R, -1, 1>(0, st, a, pa, starts, nctx, nblocks, tinc, 0, 0, s);
@@ -1831,7 +1863,7 @@ __global__ void step_advance_kernel(int *step)
 
 void launch_step_advance(int *step, hipStream_t s) { step_advance_kernel<<<1, 64, 0, s>>>(step); }
 
-int stage_tile_rows(bool fp64) { return fp64 ? lds_ty<double>() : lds_ty<float>(); }
+int stage_tile_rows(bool fp64, int tw) { return fp64 ? lds_ty<double>() : lds_ty<float>() * (tw == 32 ? 2 : 1); }
 
 // batched trace gathers: blockIdx.y = grid, whose RecArgs block is
 // kCtxArgBytes from the previous grid's
@@ -1873,8 +1905,8 @@ void launch_step_advance_batch(const char *pa, int nctx, hipStream_t s)
 #define CGFD_INSTANTIATE(R)                                                                                   \
     template void launch_stage<R>(int, int, int, int, const StageArgs<R> &, const StageArgs<R> *, int, int, int, \
                                    hipStream_t);                                                              \
-    template void launch_stage_batch<R>(int, int, int, int, int, const StageArgs<R> *, const int *, int, int, int, \
-                                        hipStream_t);                                                         \
+    template void launch_stage_batch<R>(int, int, int, int, int, int, const StageArgs<R> *, const int *, int, int, \
+                                        int, hipStream_t);                                                    \
     template void launch_timg<R>(int, int, int, int, long, const TimgArgs<R> &, const TimgArgs<R> *, int, int, \
                                   hipStream_t);                                                               \
     template void launch_record_batch<R>(const RecArgs<R> *, int, int, hipStream_t);                          \

@@ -170,7 +170,9 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
 /* 1 if cgfd_run_batch runs this list as one launch per kernel (no reference
  * counterpart): 2 to 4096 distinct contexts on one device with the same
  * precision, the default stage variant, the same metric mode and the same
- * step phase (step % 4), none timing its kernels.  Grid sizes, media, PML,
+ * step phase (step % 4), none timing its kernels, and no two given different
+ * tile widths (the batch runs the width a grid was given, else 32 when
+ * 64-column tiles need over 8 % more tiles over all grids).  Grid sizes, media, PML,
  * sources, receivers and snapshots may differ.  Such a batch runs on the
  * first context's stream, which waits for the others' queued work; a grid
  * keeps the z-chunk count cgfd_autotune or cgfd_set_tuning gave it, the
@@ -179,10 +181,12 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
  * list). */
 int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few
- * z-chunk counts around the default and keep the fastest (*chosen); with
- * trial_steps >= 4 and nt >= 8, also time 4 eager steps against 4 replayed
- * ones and switch graph replay (cgfd_set_graph) on if it is faster.  Then
- * re-initialise the context.  Call right after cgfd_init. */
+ * z-chunk counts around the default and keep the fastest (*chosen); in FP32
+ * then the other tile width of the default stage kernel (cgfd_set_tile_width)
+ * at that count, kept if faster; with trial_steps >= 4 and nt >= 8, also
+ * time 4 eager steps against 4 replayed ones and switch graph replay
+ * (cgfd_set_graph) on if it is faster.  Then re-initialise the context.
+ * Call right after cgfd_init. */
 int cgfd_autotune(cgfd_ctx *ctx, int trial_steps, int *chosen);
 /* Graph replay (no reference counterpart; for small grids whose steps are
  * bound by the gaps between dependent launches): with on != 0, cgfd_run
@@ -260,6 +264,15 @@ int cgfd_profile_read(cgfd_ctx *ctx, double *ms_stage, long *n_stage, double *ms
 /* Launch shape of the fused stage kernel: z-chunks per column sweep and
  * rows per workgroup (0 keeps the current value). */
 int cgfd_set_tuning(cgfd_ctx *ctx, int zchunks, int rows);
+/* Tile width of the default (LDS) stage kernel, no reference counterpart:
+ * 64 columns x 4 rows per workgroup, or 32 x 8, which leaves fewer lanes idle
+ * on rows whose length is 1-32 past a multiple of 64 (80 cells: 96 lanes
+ * instead of 128) but costs about 6 % more per lane.  0 (the default) takes
+ * 32 when 64-column tiles need over 8 % more lanes; FP64 contexts take 0 or
+ * 64.  Results do not depend on it.
+ * cgfd_tile_width returns the width the next launch uses. */
+int cgfd_set_tile_width(cgfd_ctx *ctx, int width);
+int cgfd_tile_width(const cgfd_ctx *ctx);
 /* Stage kernel variant: 0 = xy planes staged in LDS (default), 1 = in-plane
  * stencil taps loaded directly through the vector caches.  Both produce
  * identical results. */

@@ -181,6 +181,12 @@ def test_unfusable_lists_are_reported_and_still_run():
     assert not cgfd3d.batch_fusable(batch)
     batch[0].profile(False)
     assert cgfd3d.batch_fusable(batch)
+    batch[0].set_tile_width(32)
+    batch[1].set_tile_width(64)
+    assert not cgfd3d.batch_fusable(batch)  # two widths given
+    batch[1].set_tile_width(0)
+    assert cgfd3d.batch_fusable(batch)  # the given width runs the batch
+    batch[0].set_tile_width(0)
     # another step phase: one stream per grid
     batch[0].run(1)
     assert not cgfd3d.batch_fusable(batch)
@@ -189,6 +195,20 @@ def test_unfusable_lists_are_reported_and_still_run():
     cgfd3d.run_batch(batch, 11)
     _check_against_single(batch, scs, 12)
     other.close()
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_fused_batch_at_each_tile_width(width):
+    """A batch of mixed sizes runs at the width one grid was given (the
+    others follow it); single runs at their own widths give the same values."""
+    nt = 9
+    scs = _scenarios(nt, sizes=[(80, 30, 20), (64, 22, 16), (20, 26, 18)])
+    batch = [_sim(sc) for sc in scs]
+    assert [b.tile_width() for b in batch] == [32, 64, 32]
+    batch[1].set_tile_width(width)
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, nt)
+    _check_against_single(batch, scs, nt)
 
 
 def test_fused_batch_keeps_each_grids_tuned_chunks():

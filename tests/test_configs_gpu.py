@@ -5,7 +5,7 @@
   wavefield components equal the oracle's.
 - 600x600x300 von Karman model (20-cell PML, 200 subfaults, 400 receivers):
   3 steps from a random wavefield and PML state, every wavefield and
-  auxiliary value, trace sample and PGV value.  This is the size whose 32-bit plane byte
+  auxiliary value, trace sample and PGV value, at both tile widths.  This is the size whose 32-bit plane byte
   offsets and 150-plane z chunks the small cases never reach.
 - A fused batch of 4 uneven hill scenarios drawn from the 96-192 x 96-192 x
   64-128 distribution: 150 steps in one cgfd_run_batch call equal each
@@ -69,13 +69,18 @@ def test_vonkarman_600x600x300_full_size():
     # surface row is active from the first stage.
     sc = scenario.config_vonkarman(nt=3)
     assert (sc["ni"], sc["nj"], sc["nk"]) == (600, 600, 300) and len(sc["moment_locs"]) == 200
+    # the engine at its own tile width for 600-cell rows and at the other one
     g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
     g.setup(sc)
+    g2 = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
+    g2.set_tile_width(64 if g.tile_width() == 32 else 32)
+    g2.setup(sc)
     o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
     o.setup(sc)
     rng = np.random.default_rng(600)
     for c, a in enumerate(scenario.random_state(g.shape, rng)):
         g.set_field(c, a)
+        g2.set_field(c, a)
         o.set_field(c, a)
         del a
     for axis in range(3):
@@ -83,21 +88,25 @@ def test_vonkarman_600x600x300_full_size():
         for c in range(9):
             a = (rng.standard_normal(n) * (1e-3 if c < 3 else 1e4)).astype(np.float32)
             g.set_aux(axis, c, a)
+            g2.set_aux(axis, c, a)
             o.set_aux(axis, c, g.get_aux(axis, c, 0))  # the engine keeps the valid positions only
     g.run(3)
+    g2.run(3)
     g.sync()
     o.run(3)
-    for c in range(9):
-        a, b = g.get_field(c), o.get_field(c)
-        assert np.isfinite(b).all() and np.abs(b).max() > 0
-        assert_same(a, b, f"von Karman field {cgfd3d.COMPONENTS[c]}")
-        del a, b
-    for axis in range(3):
+    for e, tw in ((g, g.tile_width()), (g2, g2.tile_width())):
         for c in range(9):
-            assert_same(g.get_aux(axis, c, 0), o.get_aux(axis, c, 0), f"von Karman aux {axis}.{c}")
-    assert_same(g.get_seismo(), o.get_seismo(), "von Karman traces")
-    assert_same(g.get_pgv(), o.get_pgv(), "von Karman PGV")
+            a, b = e.get_field(c), o.get_field(c)
+            assert np.isfinite(b).all() and np.abs(b).max() > 0
+            assert_same(a, b, f"von Karman field {cgfd3d.COMPONENTS[c]}, tile width {tw}")
+            del a, b
+        for axis in range(3):
+            for c in range(9):
+                assert_same(e.get_aux(axis, c, 0), o.get_aux(axis, c, 0), f"von Karman aux {axis}.{c}, tile width {tw}")
+        assert_same(e.get_seismo(), o.get_seismo(), f"von Karman traces, tile width {tw}")
+        assert_same(e.get_pgv(), o.get_pgv(), f"von Karman PGV, tile width {tw}")
     g.close()
+    g2.close()
     o.close()
 
 

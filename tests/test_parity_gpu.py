@@ -17,13 +17,19 @@ import oracle  # noqa: E402
 import scenario  # noqa: E402
 
 
-VARIANTS = [0, 1, 2]  # LDS-staged planes, direct in-plane loads, split velocity/stress halves
+# LDS-staged planes (at the tile width the grid's rows pick, and at 64 x 4 and
+# 32 x 8 tiles), direct in-plane loads, split velocity/stress halves
+VARIANTS = [0, "0w64", "0w32", 1, 2]
 
 
 def _pair(sc, double=False, variant=0):
     prec = cgfd3d.FP64 if double else cgfd3d.FP32
     g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], precision=prec)
-    g.set_variant(variant)
+    if isinstance(variant, str):
+        g.set_variant(0)
+        g.set_tile_width(int(variant[2:]))
+    else:
+        g.set_variant(variant)
     o = oracle.Oracle(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"], double=double)
     if double:
         sc = dict(sc)
@@ -203,22 +209,46 @@ def test_minimum_depth_grid(variant):
 def test_tuning_does_not_change_results():
     sc = scenario.config_random(66, 30, 40, 16, [5, 5, 4, 4, 6, 0], seed=9)
     ref = None
-    for variant, zc, rows in [(1, 1, 4), (1, 3, 2), (1, 7, 1), (1, 40, 4), (0, 1, 4), (0, 5, 4), (0, 40, 4),
-                              (2, 1, 4), (2, 6, 4), (2, 40, 4)]:
+    for variant, zc, rows, tw in [(1, 1, 4, 0), (1, 3, 2, 0), (1, 7, 1, 0), (1, 40, 4, 0), (0, 1, 4, 0),
+                                  (0, 5, 4, 0), (0, 40, 4, 0), (0, 1, 4, 64), (0, 5, 4, 64), (0, 40, 4, 64),
+                                  (0, 3, 4, 32), (0, 40, 4, 32), (2, 1, 4, 0), (2, 6, 4, 0), (2, 40, 4, 0)]:
         g = cgfd3d.Simulation(sc["ni"], sc["nj"], sc["nk"], sc["dh"], sc["dt"], sc["nt"])
         g.set_variant(variant)
         g.set_tuning(zc, rows)
+        g.set_tile_width(tw)
         g.setup(sc)
         g.run(16)
         out = np.stack([g.get_field(c) for c in range(9)])
         if ref is None:
             ref = out
         else:
-            assert_same(out, ref, f"variant={variant} zchunks={zc} rows={rows}")
+            assert_same(out, ref, f"variant={variant} zchunks={zc} rows={rows} tile width={tw}")
         g.close()
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+def test_tile_width_choice_and_refusals():
+    """Rows that need over 8 % more lanes in 64-column tiles than in
+    32-column ones take 32; a width can be set (0 returns to the automatic
+    choice); FP64 runs 64."""
+    for ni, want in [(64, 64), (80, 32), (100, 64), (192, 64), (200, 32), (600, 64), (33, 64), (17, 32),
+                     (280, 32), (300, 64)]:
+        g = cgfd3d.Simulation(ni, 8, 6, 100.0, 0.005, 4)
+        assert g.tile_width() == want, ni
+        g.set_tile_width(64)
+        assert g.tile_width() == 64
+        g.set_tile_width(0)
+        assert g.tile_width() == want
+        with pytest.raises(cgfd3d.CgfdError, match="tile width"):
+            g.set_tile_width(48)
+        g.close()
+    d = cgfd3d.Simulation(80, 8, 6, 100.0, 0.005, 4, precision=cgfd3d.FP64)
+    assert d.tile_width() == 64
+    with pytest.raises(cgfd3d.CgfdError, match="64-column"):
+        d.set_tile_width(32)
+    d.close()
+
+
+@pytest.mark.parametrize("variant", [0, "0w32", 1, 2])
 def test_flat_config_full_run(variant):
     """The small flat homogeneous configuration at full length (64^3, 600
     steps): receivers, snapshots every 100 steps and PGV."""
@@ -237,7 +267,7 @@ def test_flat_config_full_run(variant):
     assert_same(g.get_pgv()[:, 3:-3, 3:-3], o.get_pgv()[:, 3:-3, 3:-3], "PGV")
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", [v for v in VARIANTS if v != "0w32"])  # FP64 runs 64-column tiles
 def test_fp64_build_matches_fp64_oracle(variant):
     sc = scenario.config_random(40, 30, 24, 30, [5, 4, 4, 5, 5, 0], seed=21, nfoc=2)
     g, o = _pair(sc, double=True, variant=variant)

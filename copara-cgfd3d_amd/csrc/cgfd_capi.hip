@@ -108,7 +108,8 @@ struct cgfd_ctx {
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     int rows = 4, zchunks = 1;
     bool zc_tuned = false; // zchunks chosen by cgfd_autotune or cgfd_set_tuning: fused batches keep it
-    int tile_w = 0;        // LDS kernel tile width set by cgfd_set_tile_width or cgfd_autotune (0: by the row length)
+    int tile_w = 0;        // LDS kernel tile width set by cgfd_set_tile_width (0: none)
+    int tile_pref = 0;     // the width cgfd_autotune found faster than the default (0: none)
     int variant = 0; // stage kernel: 0 = LDS-staged planes, 1 = direct loads
     char *args_d = nullptr; // argument image (kCtxArgBytes: stage, traction-image and trace blocks)
     bool args_dirty = true;
@@ -230,6 +231,7 @@ static int tile_width(const cgfd_ctx *c)
 {
     if (c->prec == CGFD_FP64) return 64;
     if (c->tile_w) return c->tile_w;
+    if (c->tile_pref) return c->tile_pref;
     return narrow_pays((c->g.ni + 63) / 64 * 64, (c->g.ni + 31) / 32 * 32) ? 32 : 64;
 }
 
@@ -248,6 +250,7 @@ static void default_tuning(cgfd_ctx *c)
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
     c->zc_tuned = false;
+    c->tile_pref = 0;
 }
 
 extern "C" {
@@ -1230,39 +1233,17 @@ int cgfd_set_graph(cgfd_ctx *c, int on)
 // precision, the LDS stage variant with the same metric mode, and the step
 // phase currT % 4 (which fixes the forward/backward pattern of every
 // stage), and no context is timing its kernels.  Grid sizes may differ.
-// The tile width of a fused batch (one kernel for all grids): the width a
-// grid was given (cgfd_set_tile_width, cgfd_autotune) if any, else 32 when
-// 64-column tiles take over 8 % more tiles over all grids.  0 if two grids
-// were given different widths.
-static int batch_width(cgfd_ctx *const *ctxs, int n)
-{
-    int given = 0;
-    long t64 = 0, t32 = 0;
-    for (int i = 0; i < n; i++) {
-        const cgfd_ctx *c = ctxs[i];
-        const int w = c->prec == CGFD_FP64 ? 64 : c->tile_w;
-        if (w) {
-            if (given && w != given) return 0;
-            given = w;
-        }
-        t64 += plane_tiles(c, 64);
-        t32 += c->prec == CGFD_FP64 ? plane_tiles(c, 64) : plane_tiles(c, 32);
-    }
-    return given ? given : (narrow_pays(t64, t32) ? 32 : 64);
-}
 
 static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 {
     if (n < 2 || n > 4096) return false; // 9 traction-image parts per grid in a 65535 grid.z
     const cgfd_ctx *c0 = ctxs[0];
     auto hm = [](const cgfd_ctx *c) { return c->hmet && c->use_hmet; };
-    const int tw = batch_width(ctxs, n);
-    if (!tw) return false; // grids given different tile widths
     long most = 0; // workgroups at the most z chunks batch_layout can pick
     for (int i = 0; i < n; i++) {
         const cgfd_ctx *c = ctxs[i];
         if (c->prof || c->variant != 0) return false;
-        most += plane_tiles(c, tw) * std::max({1, c->g.nk / 2, c->zc_tuned ? c->zchunks : 0});
+        most += plane_tiles(c, tile_width(c)) * std::max({1, c->g.nk / 2, c->zc_tuned ? c->zchunks : 0});
         if (i == 0) continue;
         if (c->dev != c0->dev || c->prec != c0->prec || c->currT % 4 != c0->currT % 4 || hm(c) != hm(c0))
             return false;
@@ -1272,9 +1253,9 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
     return most < 0x7fffffffL;
 }
 
-// Workgroups of a batched stage launch: every grid sweeps its columns in
-// about the same number of z chunks, chosen for about CGFD_BATCH_WORKGROUPS
-// workgroups (tiles of the batch's width tw) over all grids, with at most
+// Workgroups of a batched stage launch (the grids of one tile width tw):
+// every grid sweeps its columns in about the same number of z chunks, chosen
+// for about CGFD_BATCH_WORKGROUPS workgroups over the launch, with at most
 // ~25 and at least 2 planes per chunk per grid; a grid whose z-chunk count
 // was tuned (cgfd_autotune, cgfd_set_tuning) keeps its own.  zlen[c] is grid
 // c's chunk length, starts[c] its first workgroup.
@@ -1302,7 +1283,7 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, int tw, int *zlen, int *s
 // address hipGetSymbolAddress returns for the current device.
 constexpr int kStaticBatchMax = 64;
 constexpr size_t kStaticBatchBytes =
-    ((size_t)kStaticBatchMax * kCtxArgBytes + (kStaticBatchMax + 1) * sizeof(int) + 255) / 256 * 256;
+    ((size_t)kStaticBatchMax * kCtxArgBytes + (kStaticBatchMax + 2) * sizeof(int) + 255) / 256 * 256;
 static __device__ __align__(256) char scratch[kStaticBatchBytes];
 
 namespace {
@@ -1335,35 +1316,55 @@ static int scratch_setup(cgfd_ctx *c, ScratchDev **out)
     return 0;
 }
 
-template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, int nsteps, ScratchDev *fixed)
+template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs_in, int n, int nsteps, ScratchDev *fixed)
 {
-    cgfd_ctx *c0 = ctxs[0];
+    cgfd_ctx *c0 = ctxs_in[0];
     HIPCHK(c0, hipSetDevice(c0->dev));
-    int zlen[kStaticBatchMax], starts_fixed[kStaticBatchMax + 1];
+    // Every grid runs at its own tile width: the grids of width 64 first,
+    // then those of width 32 (group 1), one stage launch per group.  The
+    // workgroup starts of each group begin at 0: group g's table lies at
+    // starts + gfirst[g] + g.
+    cgfd_ctx *ord_fixed[kStaticBatchMax];
+    int zlen[kStaticBatchMax], starts_fixed[kStaticBatchMax + 2];
+    std::vector<cgfd_ctx *> ov;
     std::vector<int> zv, sv;
+    cgfd_ctx **ctxs = ord_fixed;
     int *zl = zlen, *starts = starts_fixed;
     if (!fixed) {
         for (int i = 0; i < n; i++)
-            if (ctxs[i]->args_dirty) {
-                int rc = upload_args(ctxs[i]);
+            if (ctxs_in[i]->args_dirty) {
+                int rc = upload_args(ctxs_in[i]);
                 if (rc) return rc;
             }
+        ov.resize(n);
         zv.resize(n);
-        sv.resize(n + 1);
+        sv.resize(n + 2);
+        ctxs = ov.data();
         zl = zv.data();
         starts = sv.data();
     }
-    const int tw = batch_width(ctxs, n); // nonzero: the list is fusable
-    batch_layout(ctxs, n, tw, zl, starts);
-    // grids that share one medium and have the same size and chunks run
-    // interleaved (workgroup = tile * n + grid): the grids' copies of one tile
-    // run together and read the shared metric and media lines from cache
-    bool interleave = n > 1;
-    for (int i = 1; i < n && interleave; i++)
-        interleave = ctxs[i]->met == c0->met && ctxs[i]->g.ni == c0->g.ni && ctxs[i]->g.nj == c0->g.nj &&
-                     ctxs[i]->g.nk == c0->g.nk && zl[i] == zl[0] &&
-                     starts[i + 1] - starts[i] == starts[1] - starts[0];
-    const size_t sofs = (size_t)n * kCtxArgBytes;
+    int gcount[2] = {0, 0};
+    for (int i = 0; i < n; i++)
+        if (tile_width(ctxs_in[i]) == 64) ctxs[gcount[0]++] = ctxs_in[i];
+    for (int i = 0; i < n; i++)
+        if (tile_width(ctxs_in[i]) != 64) ctxs[gcount[0] + gcount[1]++] = ctxs_in[i];
+    const int gfirst[2] = {0, gcount[0]}, gwidth[2] = {64, 32};
+    bool interleave[2] = {false, false};
+    for (int g = 0; g < 2; g++) {
+        if (!gcount[g]) continue;
+        cgfd_ctx *const *gc = ctxs + gfirst[g];
+        int *gs = starts + gfirst[g] + g, *gz = zl + gfirst[g];
+        const int m = gcount[g];
+        batch_layout(gc, m, gwidth[g], gz, gs);
+        // grids that share one medium and have the same size and chunks run
+        // interleaved (the copies of one tile together on one XCD) and read
+        // the shared metric and media lines from its L2
+        interleave[g] = m > 1;
+        for (int i = 1; i < m && interleave[g]; i++)
+            interleave[g] = gc[i]->met == gc[0]->met && gc[i]->g.ni == gc[0]->g.ni && gc[i]->g.nj == gc[0]->g.nj &&
+                            gc[i]->g.nk == gc[0]->g.nk && gz[i] == gz[0] && gs[i + 1] - gs[i] == gs[1] - gs[0];
+    }
+    const size_t sofs = (size_t)n * kCtxArgBytes, sbytes = (size_t)(n + 2) * sizeof(int);
     int max_nrecv = 0, ni = 0, nj = 0;
     long cols = 0;
     for (int i = 0; i < n; i++) {
@@ -1384,13 +1385,13 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             make_image<R>(ctxs[i], one, zl[i]);
             HIPCHK(c0, hipMemcpy(fixed->ptr + (size_t)i * kCtxArgBytes, one, kCtxArgBytes, hipMemcpyHostToDevice));
         }
-        HIPCHK(c0, hipMemcpy(fixed->ptr + sofs, starts, (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c0, hipMemcpy(fixed->ptr + sofs, starts, sbytes, hipMemcpyHostToDevice));
         img_d = fixed->ptr;
     } else {
         // device image: the grids' argument images, then the workgroup starts
-        std::vector<char> img(sofs + (n + 1) * sizeof(int));
+        std::vector<char> img(sofs + sbytes);
         for (int i = 0; i < n; i++) make_image<R>(ctxs[i], img.data() + (size_t)i * kCtxArgBytes, zl[i]);
-        memcpy(img.data() + sofs, starts, (n + 1) * sizeof(int));
+        memcpy(img.data() + sofs, starts, sbytes);
         if (img != c0->bimg_h) {
             // launches of an earlier batch on c0's stream may still read the image
             HIPCHK(c0, hipStreamSynchronize(c0->stream));
@@ -1408,7 +1409,7 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
     }
     // c0's stream runs the batch: it waits for work queued on the others
     for (int i = 1; i < n; i++) {
-        cgfd_ctx *c = ctxs[i];
+        cgfd_ctx *c = ctxs_in[i];
         HIPCHK(c, hipEventRecord(c->bev, c->stream));
         HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->bev, 0));
     }
@@ -1423,8 +1424,15 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
             const int tinc = k == 0 ? 0 : (k < 3 ? 1 : 2);
             launch_timg<R>(sx, sz, ni, nj, cols, TimgArgs<R>{},
                            (const TimgArgs<R> *)(img_d + kTimgArgsOffset + k * kTimgArgsBytes), n, 4, c0->stream);
-            launch_stage_batch<R>(sx, sz, k, hmet, hybrid, tw, (const StageArgs<R> *)(img_d + k * kStageArgsBytes),
-                                  starts_d, interleave ? -n : n, starts[n], tinc, c0->stream);
+            for (int g = 0; g < 2; g++) {
+                if (!gcount[g]) continue;
+                const int fo = gfirst[g] + g; // this group's starts table
+                launch_stage_batch<R>(sx, sz, k, hmet, hybrid, gwidth[g],
+                                      (const StageArgs<R> *)(img_d + (size_t)gfirst[g] * kCtxArgBytes +
+                                                             k * kStageArgsBytes),
+                                      starts_d + fo, interleave[g] ? -gcount[g] : gcount[g], starts[fo + gcount[g]],
+                                      tinc, c0->stream);
+            }
         }
         launch_record_batch<R>((const RecArgs<R> *)(img_d + kRecArgsOffset), n, max_nrecv, c0->stream);
         for (int i = 0; i < n; i++) {
@@ -1442,7 +1450,7 @@ template <typename R> static int run_batch_fused(cgfd_ctx *const *ctxs, int n, i
     }
     // the other streams wait for the batch before their own later work
     HIPCHK(c0, hipEventRecord(c0->bev, c0->stream));
-    for (int i = 1; i < n; i++) HIPCHK(ctxs[i], hipStreamWaitEvent(ctxs[i]->stream, c0->bev, 0));
+    for (int i = 1; i < n; i++) HIPCHK(ctxs_in[i], hipStreamWaitEvent(ctxs_in[i]->stream, c0->bev, 0));
     if (fixed) {
         HIPCHK(c0, hipEventRecord(fixed->done, c0->stream));
         fixed->pending = true;
@@ -1584,13 +1592,13 @@ int cgfd_autotune(cgfd_ctx *c, int trial_steps, int *chosen)
     c->zchunks = best;
     c->zc_tuned = true;
     c->args_dirty = true;
-    // then the LDS kernel's other tile width (FP32) at those chunks
-    if (!rc && c->prec == CGFD_FP32 && c->variant == 0) {
-        const int w0 = tile_width(c);
-        c->tile_w = w0 == 64 ? 32 : 64;
+    // then the LDS kernel's other tile width (FP32) at those chunks, unless
+    // cgfd_set_tile_width fixed one
+    if (!rc && c->prec == CGFD_FP32 && c->variant == 0 && c->tile_w == 0) {
+        c->tile_pref = tile_width(c) == 64 ? 32 : 64;
         float ms = 0;
         rc = trial(&ms);
-        if (rc || !(ms < best_ms)) c->tile_w = w0;
+        if (rc || !(ms < best_ms)) c->tile_pref = 0;
         c->args_dirty = true;
     }
     // then eager stepping against graph replay of 4-step cycles (one warm-up

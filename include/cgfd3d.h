@@ -170,9 +170,8 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
 /* 1 if cgfd_run_batch runs this list as one launch per kernel (no reference
  * counterpart): 2 to 4096 distinct contexts on one device with the same
  * precision, the default stage variant, the same metric mode and the same
- * step phase (step % 4), none timing its kernels, and no two given different
- * tile widths (the batch runs the width a grid was given, else 32 when
- * 64-column tiles need over 8 % more tiles over all grids).  Grid sizes, media, PML,
+ * step phase (step % 4), none timing its kernels.  Each grid runs at its
+ * own tile width (cgfd_tile_width): one stage launch per width.  Grid sizes, media, PML,
  * sources, receivers and snapshots may differ.  Such a batch runs on the
  * first context's stream, which waits for the others' queued work; a grid
  * keeps the z-chunk count cgfd_autotune or cgfd_set_tuning gave it, the
@@ -182,8 +181,9 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
 int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);
 /* Autotuner (no reference counterpart): time trial_steps steps for a few
  * z-chunk counts around the default and keep the fastest (*chosen); in FP32
- * then the other tile width of the default stage kernel (cgfd_set_tile_width)
- * at that count, kept if faster; with trial_steps >= 4 and nt >= 8, also
+ * and unless cgfd_set_tile_width fixed one, then the other tile width of the
+ * default stage kernel at that count, kept if faster; with trial_steps >= 4
+ * and nt >= 8, also
  * time 4 eager steps against 4 replayed ones and switch graph replay
  * (cgfd_set_graph) on if it is faster.  Then re-initialise the context.
  * Call right after cgfd_init. */

@@ -183,10 +183,7 @@ def test_unfusable_lists_are_reported_and_still_run():
     assert cgfd3d.batch_fusable(batch)
     batch[0].set_tile_width(32)
     batch[1].set_tile_width(64)
-    assert not cgfd3d.batch_fusable(batch)  # two widths given
-    batch[1].set_tile_width(0)
-    assert cgfd3d.batch_fusable(batch)  # the given width runs the batch
-    batch[0].set_tile_width(0)
+    assert cgfd3d.batch_fusable(batch)  # one stage launch per tile width
     # another step phase: one stream per grid
     batch[0].run(1)
     assert not cgfd3d.batch_fusable(batch)
@@ -199,8 +196,8 @@ def test_unfusable_lists_are_reported_and_still_run():
 
 @pytest.mark.parametrize("width", [32, 64])
 def test_fused_batch_at_each_tile_width(width):
-    """A batch of mixed sizes runs at the width one grid was given (the
-    others follow it); single runs at their own widths give the same values."""
+    """Every grid of a fused batch runs at its own tile width (one stage
+    launch per width, here one or two); single runs give the same values."""
     nt = 9
     scs = _scenarios(nt, sizes=[(80, 30, 20), (64, 22, 16), (20, 26, 18)])
     batch = [_sim(sc) for sc in scs]
@@ -290,3 +287,23 @@ def test_shared_medium_refuses_what_would_change_it():
     a.set_metrics(scs[0]["metrics"])
     for g in (a, other, fp64, empty, fresh):
         g.close()
+
+
+def test_two_width_groups_each_on_one_shared_medium():
+    """A list mixing two media whose rows take different tile widths (80
+    cells: 32, 64 cells: 64), listed alternately: one stage launch per width,
+    each interleaving its grids' copies of a tile; every grid equals its
+    single run."""
+    nt = 8
+    a = _on_one_medium(2, (80, 30, 20), nt)
+    b = _on_one_medium(2, (64, 22, 16), nt)
+    ga = [_sim(a[0])]
+    ga.append(_sim(dict(a[1], medium_of=ga[0])))
+    gb = [_sim(b[0])]
+    gb.append(_sim(dict(b[1], medium_of=gb[0])))
+    assert [g.tile_width() for g in ga + gb] == [32, 32, 64, 64]
+    batch = [ga[0], gb[0], ga[1], gb[1]]
+    assert cgfd3d.batch_fusable(batch)
+    cgfd3d.run_batch(batch, 5)
+    cgfd3d.run_batch(batch, nt - 5)
+    _check_against_single(batch, [a[0], b[0], a[1], b[1]], nt)

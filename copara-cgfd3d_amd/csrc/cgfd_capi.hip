@@ -213,8 +213,8 @@ static void fb_signs(int t, int k, int *sx, int *sz)
     *sz = (((p == 1 || p == 2) ? 1 : 0) ^ (k & 1)) ? -1 : 1;
 }
 
-// z chunks per column sweep: enough workgroups to fill the chip (~2048 of
-// 64x4 tiles) and at most ~25 planes per chunk, but at least 2 planes per
+// z chunks per column sweep: enough workgroups to fill the chip (~2048
+// 256-cell tiles) and at most ~25 planes per chunk, but at least 2 planes per
 // chunk.  Small grids want many short chunks (64^3: 2.88 -> 4.73
 // Gcell-RKstages/s going from 8 to 2 planes); short chunks also keep the
 // launch's tail short (600x600x300: 6.07 ms per stage with 2 chunks of 150

@@ -665,7 +665,8 @@ __global__ void __launch_bounds__(256) stage_direct_kernel(const StageArgs<R> *p
 }
 
 // Variant "lds": xy-plane staging.  A workgroup of LDS_TY wavefronts owns a
-// 64 x LDS_TY tile of (x,y) columns and sweeps a chunk of z planes.  Per plane
+// 64 x LDS_TY (or 32 x 2*LDS_TY) tile of (x,y) columns and sweeps a chunk of
+// z planes.  Per plane
 // every lane writes its cell of the 9 stage-input components into an LDS tile
 // that also holds the one-sided halo (1 cell behind and 3 ahead for forward
 // differences, 3 behind and 1 ahead for backward), so all 18 in-plane stencils

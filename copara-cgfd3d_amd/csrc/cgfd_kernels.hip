@@ -1435,10 +1435,33 @@ __global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> av, const T
     // The 3 top rows x 3 momentum equations are independent.  Small grids
     // split them over blockIdx.z (SPLIT = 9: one pair per block; 3: one row
     // per block) so a thread does not run nine dependent load chains; large
-    // grids keep all nine per thread (SPLIT = 1), which shares the loads of a
-    // column.  The loops stay unrolled, so r and m are compile-time constants
-    // in every copy.
+    // grids keep all nine per thread (SPLIT = 1).  With SPLIT 1 or 3 a point's
+    // metrics and six stresses are loaded once for the three momentum rows,
+    // and the zeta traction vectors of the planes below the surface once for
+    // all rows (the products are the same, so are the results).  The loops
+    // stay unrolled, so r and m are compile-time constants in every copy.
     const int rz = SPLIT == 9 ? bz / 3 : bz, mz = bz % 3;
+    constexpr int TS[6] = {TXX, TYY, TZZ, TXY, TXZ, TYZ};
+    // row m of the traction in terms of TS: (Txx,Txy,Txz), (Txy,Tyy,Tyz), (Txz,Tyz,Tzz)
+    constexpr int tsel[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    // J * (d . T_m) for the 3 rows m at point q, d the metric triple at slot d0
+    auto traction = [&](long q, int d0, R (&out)[3]) {
+        R t[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) t[u] = a->win[TS[u] * fs + q];
+        const R j = jac[q], m0 = a->met[d0 * fs + q], m1 = a->met[(d0 + 1) * fs + q], m2 = a->met[(d0 + 2) * fs + q];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) out[m] = j * (m0 * t[tsel[m][0]] + m1 * t[tsel[m][1]] + m2 * t[tsel[m][2]]);
+    };
+    // zeta traction vectors of the planes nk-6+r0 .. nk-2 (the surface plane's
+    // own traction is zero and never read)
+    R tz[5][3];
+    const int r0 = SPLIT == 3 ? rz : 0;
+    if (SPLIT != 9) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            if (j < 5 - r0) traction(gidx(g, x, y, g.nk - 6 + r0 + j), ZTX, tz[j]);
+    }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         if (SPLIT > 1 && r != rz) continue;
@@ -1446,38 +1469,60 @@ __global__ void __launch_bounds__(256) timg_kernel(const TimgArgs<R> av, const T
         const long G = gidx(g, x, y, z);
         const R rhojac = (R)(1.0 / (double)a->med[3 * fs + G] / (double)jac[G]);
         const long tcell = ((long)r * g.nj + y) * row + x;
+        const int s = 3 + (2 - r); // the surface in the z stencil
+        R vx[3][7], vy[3][7], vz[3][7];
+        if (SPLIT != 9) {
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                R ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0};
+                if ((SX > 0 && i >= 2) || (SX < 0 && i <= 4)) {
+                    traction(G + (i - 3), XIX, ox);
+                    traction(G + (i - 3) * row, ETX, oy);
+                }
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    vx[m][i] = ox[m];
+                    vy[m][i] = oy[m];
+                    // plane z-3+i is tz slot i (SPLIT 3: r0 = r) or r + i
+                    vz[m][i] = i < s ? tz[SPLIT == 3 ? i : r + i][m] : (R)0;
+                }
+            }
+        }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             if (SPLIT == 9 && m != mz) continue;
-            const R *T0 = a->win + trow[m][0] * fs;
-            const R *T1 = a->win + trow[m][1] * fs;
-            const R *T2 = a->win + trow[m][2] * fs;
-            R vx[7], vy[7], vz[7];
+            if (SPLIT == 9) {
+                const R *T0 = a->win + trow[m][0] * fs;
+                const R *T1 = a->win + trow[m][1] * fs;
+                const R *T2 = a->win + trow[m][2] * fs;
 #pragma unroll
-            for (int i = 0; i < 7; ++i) {
-                vx[i] = vy[i] = 0;
-                if ((SX > 0 && i >= 2) || (SX < 0 && i <= 4)) {
-                    long q = G + (i - 3);
-                    vx[i] = jac[q] * (a->met[XIX * fs + q] * T0[q] + a->met[XIY * fs + q] * T1[q] +
-                                      a->met[XIZ * fs + q] * T2[q]);
-                    q = G + (i - 3) * row;
-                    vy[i] = jac[q] * (a->met[ETX * fs + q] * T0[q] + a->met[ETY * fs + q] * T1[q] +
-                                      a->met[ETZ * fs + q] * T2[q]);
+                for (int i = 0; i < 7; ++i) {
+                    vx[m][i] = vy[m][i] = 0;
+                    if ((SX > 0 && i >= 2) || (SX < 0 && i <= 4)) {
+                        long q = G + (i - 3);
+                        vx[m][i] = jac[q] * (a->met[XIX * fs + q] * T0[q] + a->met[XIY * fs + q] * T1[q] +
+                                             a->met[XIZ * fs + q] * T2[q]);
+                        q = G + (i - 3) * row;
+                        vy[m][i] = jac[q] * (a->met[ETX * fs + q] * T0[q] + a->met[ETY * fs + q] * T1[q] +
+                                             a->met[ETZ * fs + q] * T2[q]);
+                    }
+                    vz[m][i] = 0;
+                    if (i < s) {
+                        const long q = G + (i - 3) * pl;
+                        vz[m][i] = jac[q] * (a->met[ZTX * fs + q] * T0[q] + a->met[ZTY * fs + q] * T1[q] +
+                                             a->met[ZTZ * fs + q] * T2[q]);
+                    }
                 }
-                const long q = G + (i - 3) * pl;
-                vz[i] = jac[q] * (a->met[ZTX * fs + q] * T0[q] + a->met[ZTY * fs + q] * T1[q] +
-                                  a->met[ZTZ * fs + q] * T2[q]);
             }
             // odd images above the surface, zero traction on it
-            const int s = 3 + (2 - r);
 #pragma unroll
             for (int j = 1; j <= 3; ++j)
-                if (j <= 3 - (2 - r)) vz[s + j] = (R)0 - vz[s - j];
-            vz[s] = 0;
+                if (j <= 3 - (2 - r)) vz[m][s + j] = (R)0 - vz[m][s - j];
+            vz[m][s] = 0;
 
-            const R DxTx = (R)((double)rhojac * (drp_sum_strf<R, SX>(vx) / hx));
-            const R DyTy = (R)((double)rhojac * (drp_sum_strf<R, SX>(vy) / hx));
-            const R DzTz = (R)((double)rhojac * (drp_sum_strf<R, SZ>(vz) / hz));
+            const R DxTx = (R)((double)rhojac * (drp_sum_strf<R, SX>(vx[m]) / hx));
+            const R DyTy = (R)((double)rhojac * (drp_sum_strf<R, SX>(vy[m]) / hx));
+            const R DzTz = (R)((double)rhojac * (drp_sum_strf<R, SZ>(vz[m]) / hz));
             R hv = (Bx == (R)1 && By == (R)1) ? DxTx + DyTy + DzTz : DxTx / Bx + DyTy / By + DzTz;
             if (inx) {
                 const R av = a->ain[0][m * a->astr[0] + ((long)z * g.nj + y) * a->npml[0] + ix];

@@ -832,7 +832,10 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     const int hy = y0 - O + hrow, hxx = x0 + tx;
     const bool hv1 = hk < 4 && hy >= -3 && hy < g.nyp;
     constexpr int NR = (NF + HALVES - 1) / HALVES;
-    auto hrfield = [&](int j) { return min(part * NR + j, NF - 1); };
+    // field of halo-row slot j; a plain expression rather than a lambda,
+    // whose call inside a field address kept the 64-column kernel's field
+    // bases out of scalar registers (36 more 64-bit vector adds)
+#define CGFD_HRFIELD(j) (TW == 64 ? (j) : min(part * NR + (j), NF - 1))
     auto hrkeep = [&](int j) { return hk < 4 && part * NR + j < NF; };
     constexpr int NH = (NF + TY * HALVES - 1) / (TY * HALVES);
     const bool xlane = tx < 4 * TR;
@@ -863,8 +866,8 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
     for (int p = z0; p < z0 + HD; ++p) {
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            const R v1 = ldg(a->win + hrfield(j) * fs0, pbase(p) + rel_1);
-            if (hrkeep(j)) lds[slot(p)][hrfield(j)][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
+            const R v1 = ldg(a->win + CGFD_HRFIELD(j) * fs0, pbase(p) + rel_1);
+            if (hrkeep(j)) lds[slot(p)][CGFD_HRFIELD(j)][hrow][O + tx] = (LT)(hv1 ? v1 : (R)0);
         }
 #pragma unroll
         for (int j = 0; j < NH; ++j) {
@@ -877,7 +880,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
         lead[c] = ldg(a->win + c * fs0, pbase(z0 + D) + rel_c);
-        if (c < NR) h1[c] = ldg(a->win + hrfield(c) * fs0, pbase(z0 + HD) + rel_1);
+        if (c < NR) h1[c] = ldg(a->win + CGFD_HRFIELD(c) * fs0, pbase(z0 + HD) + rel_1);
     }
 #pragma unroll
     for (int j = 0; j < NH; ++j) h2[j] = ldg(a->win + hfield(j) * fs0, pbase(z0 + HD) + rel_2);
@@ -905,7 +908,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             lds[sc][c][O + ty][O + tx] = (LT)w[c][CI];
-            if (c < NR && hrkeep(c)) lds[sh][hrfield(c)][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
+            if (c < NR && hrkeep(c)) lds[sh][CGFD_HRFIELD(c)][hrow][O + tx] = (LT)(hv1 ? h1[c] : (R)0);
             w[c][4] = lead[c];
         }
 #pragma unroll
@@ -917,7 +920,7 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
             lead[c] = ldg(al->win + c * fs, pbase(z + 1 + D) + rel_c);
-            if (c < NR) h1[c] = ldg(al->win + hrfield(c) * fs, pbase(z + 1 + HD) + rel_1);
+            if (c < NR) h1[c] = ldg(al->win + CGFD_HRFIELD(c) * fs, pbase(z + 1 + HD) + rel_1);
         }
 #pragma unroll
         for (int j = 0; j < NH; ++j) h2[j] = ldg(al->win + hfield(j) * fs, pbase(z + 1 + HD) + rel_2);
@@ -955,6 +958,8 @@ stage_lds_kernel(const StageArgs<R> *pa, const int *starts, int nctx, int tinc)
         }
     }
 }
+
+#undef CGFD_HRFIELD
 
 // ---------------------------------------------------------------------------
 // Variant "split": each stage as two kernels, one per half of the system.

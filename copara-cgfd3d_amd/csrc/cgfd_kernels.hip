@@ -1851,7 +1851,16 @@ void launch_timg(int sx, int sz, int ni, int nj, long cols, const TimgArgs<R> &a
                  int rows, hipStream_t s)
 {
     dim3 block(64, rows);
-    const int split = cols < 65536 ? 9 : (cols < 131072 ? 3 : 1); // see timg_kernel
+// Columns below which the 9 row equations get a block each, and below which
+// each of the 3 rows gets one (measured per stage: 64^3 8.6 us at 9 parts,
+// 10.8 at 3; 200x200 15.1 us at 3, 16.3 at 9; 600x600 52 us at 1 or 3).
+#ifndef CGFD_TIMG_SPLIT9
+#define CGFD_TIMG_SPLIT9 16384
+#endif
+#ifndef CGFD_TIMG_SPLIT3
+#define CGFD_TIMG_SPLIT3 131072
+#endif
+    const int split = cols < CGFD_TIMG_SPLIT9 ? 9 : (cols < CGFD_TIMG_SPLIT3 ? 3 : 1); // see timg_kernel
     dim3 grid((ni + 63) / 64, (nj + rows - 1) / rows, split * nctx);
     if (split == 9) launch_timg_split<R, 9>(sx, sz, av, pa, grid, block, s);
     else if (split == 3) launch_timg_split<R, 3>(sx, sz, av, pa, grid, block, s);

@@ -130,9 +130,9 @@ int cgfd_set_simplecar(cgfd_ctx *ctx, int on);
  * surrogate's training runs).  Both must be on one device, of one precision
  * and grid size; src must have its metrics and media, ctx none yet.  The
  * arrays are freed with the last context that holds them; while more than
- * one does, none of them may set metrics or media.  A fused batch of such
- * grids with one z-chunk length runs their copies of each tile together, so
- * the shared lines are read from cache. */
+ * one does, none of them may set metrics or media.  In a fused batch, such
+ * grids of one tile width and z-chunk length run their copies of each tile
+ * together, so the shared lines are read from cache. */
 int cgfd_share_medium(cgfd_ctx *ctx, const cgfd_ctx *src);
 
 /* VelCoeff (calculate.cu:1336-1353) plus state reset: zero the wavefield,
@@ -171,11 +171,11 @@ int cgfd_run_batch_static(cgfd_ctx *const *ctxs, int n, int nsteps);
  * counterpart): 2 to 4096 distinct contexts on one device with the same
  * precision, the default stage variant, the same metric mode and the same
  * step phase (step % 4), none timing its kernels.  Each grid runs at its
- * own tile width (cgfd_tile_width): one stage launch per width.  Grid sizes, media, PML,
- * sources, receivers and snapshots may differ.  Such a batch runs on the
- * first context's stream, which waits for the others' queued work; a grid
- * keeps the z-chunk count cgfd_autotune or cgfd_set_tuning gave it, the
- * others get the batch's own (about 2048 workgroups over all grids); the
+ * own tile width (cgfd_tile_width): one stage launch per width.  Grid sizes,
+ * media, PML, sources, receivers and snapshots may differ.  Such a batch
+ * runs on the first context's stream, which waits for the others' queued
+ * work; a grid keeps the z-chunk count cgfd_autotune or cgfd_set_tuning gave
+ * it, the others get the batch's own (about 2048 workgroups per launch); the
  * other streams wait for the batch.  Otherwise 0 (or CGFD_EINVAL for a null
  * list). */
 int cgfd_batch_fusable(cgfd_ctx *const *ctxs, int n);

@@ -71,6 +71,10 @@ def test_batch_entry_points_validate_without_the_gpu():
     assert lib.cgfd_run_batch(empty, 2, 1) == -1
     assert lib.cgfd_batch_fusable(empty, 0) == 0                # an empty list is not fusable
     assert lib.cgfd_batch_fusable(empty, -1) == -1
+    # the shared medium and tile width entry points refuse null contexts
+    assert lib.cgfd_share_medium(None, None) == -1
+    assert lib.cgfd_set_tile_width(None, 32) == -1
+    assert lib.cgfd_tile_width(None) == -1
 
 
 FNO_HEADER = os.path.join(ROOT, "include", "cgfd_fno.h")

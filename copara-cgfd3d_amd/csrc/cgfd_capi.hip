@@ -214,12 +214,13 @@ static void fb_signs(int t, int k, int *sx, int *sz)
 }
 
 // z chunks per column sweep: enough workgroups to fill the chip (~2048
-// 256-cell tiles) and at most ~25 planes per chunk, but at least 2 planes per
+// 256-cell tiles) and at most ~15 planes per chunk, but at least 2 planes per
 // chunk.  Small grids want many short chunks (64^3: 2.88 -> 4.73
 // Gcell-RKstages/s going from 8 to 2 planes); short chunks also keep the
 // launch's tail short (600x600x300: 6.07 ms per stage with 2 chunks of 150
-// planes, 5.59 ms with 10 of 30, 5.30 ms with 12 of 25 after later changes;
-// 200x200x150 is fastest at 11 chunks of 14 planes).
+// planes, 5.59 ms with 10 of 30; with later kernels 5.338 ms at 10 chunks,
+// 5.328 at 12, 5.305 at 15, 5.300 at 20 and 5.313 at 25; 200x200x150 is
+// fastest at 12 chunks of 13 planes).
 // Tile width of the LDS stage kernel: the one set or tuned, else 32 when a
 // row of ni cells takes over 8 % more lanes in 64-column tiles than in
 // 32-column ones.  Measured per stage: 80 cells (128 vs 96 lanes) 13 % and
@@ -246,7 +247,7 @@ static void default_tuning(cgfd_ctx *c)
 {
     c->rows = 4;
     const long tiles = plane_tiles(c, tile_width(c));
-    long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 24) / 25);
+    long zc = std::max((2048 + tiles - 1) / tiles, (long)(c->g.nk + 14) / 15);
     zc = std::max(1L, std::min(zc, (long)std::max(1, c->g.nk / 2)));
     c->zchunks = (int)zc;
     c->zc_tuned = false;

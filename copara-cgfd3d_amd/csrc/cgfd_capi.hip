@@ -1263,6 +1263,9 @@ static bool batch_fusable(cgfd_ctx *const *ctxs, int n)
 #ifndef CGFD_BATCH_WORKGROUPS
 #define CGFD_BATCH_WORKGROUPS 2048
 #endif
+#ifndef CGFD_BATCH_PLANES
+#define CGFD_BATCH_PLANES 25 // at most about this many planes per chunk
+#endif
 static void batch_layout(cgfd_ctx *const *ctxs, int n, int tw, int *zlen, int *starts)
 {
     long tiles = 0;
@@ -1272,7 +1275,9 @@ static void batch_layout(cgfd_ctx *const *ctxs, int n, int tw, int *zlen, int *s
     for (int i = 0; i < n; i++) {
         const int nk = ctxs[i]->g.nk;
         const int zci = ctxs[i]->zc_tuned ? std::max(1, std::min(ctxs[i]->zchunks, nk))
-                                          : (int)std::min(std::max(zc, (long)(nk + 24) / 25), (long)std::max(1, nk / 2));
+                                          : (int)std::min(std::max(zc, (long)(nk + CGFD_BATCH_PLANES - 1) /
+                                                                              CGFD_BATCH_PLANES),
+                                                          (long)std::max(1, nk / 2));
         zlen[i] = (nk + zci - 1) / zci;
         starts[i + 1] = starts[i] + (int)(plane_tiles(ctxs[i], tw) * zci);
     }
